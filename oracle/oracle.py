@@ -1,0 +1,147 @@
+"""ctypes loader for the CPU oracle (``liboracle_pdlp.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, the smoke entry point and the
+``cpu_baseline`` leg of ``bench.py``, never by the ``sovlp`` package.
+
+``solve(prob, ...)`` restates ``solve_pdlp`` (reference ``src/sovopt/lp/pdlp.py:238-494``)
+and ``scale(prob)`` restates ``scale_problem(..., "pdlp")``
+(``src/sovopt/numerics/scaling.py:374-478``); see ``pdlp_oracle.c`` for the line map.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_pdlp.so")
+
+_lib = None
+
+
+class _Params(C.Structure):
+    _fields_ = [("eps_abs", C.c_double), ("eps_rel", C.c_double), ("feas_cap", C.c_double),
+                ("max_iter", C.c_longlong), ("time_limit", C.c_double),
+                ("check_every", C.c_int), ("stepsize_every", C.c_int),
+                ("beta_sufficient", C.c_double), ("beta_necessary", C.c_double),
+                ("beta_artificial", C.c_double), ("primal_weight_smoothing", C.c_double)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong), ("restarts", C.c_longlong),
+                ("rejected_steps", C.c_longlong), ("primal_res", C.c_double),
+                ("dual_res", C.c_double), ("gap", C.c_double), ("primal_obj", C.c_double),
+                ("dual_obj", C.c_double), ("step_size", C.c_double),
+                ("primal_weight", C.c_double), ("objective", C.c_double),
+                ("scale_obj", C.c_double), ("history_len", C.c_longlong)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH) or \
+            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "pdlp_oracle.c")):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "liboracle_pdlp.so"],
+                       check=True, capture_output=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.oracle_pdlp_solve.restype = C.c_int
+        L.oracle_pdlp_solve.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P,
+                                        C.c_double, C.c_int, P, C.POINTER(_Params), P, P, P,
+                                        C.POINTER(_Info), P, C.c_longlong, C.c_int]
+        L.oracle_scale.restype = C.c_int
+        L.oracle_scale.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P]
+        L.oracle_spmv.restype = None
+        L.oracle_spmv.argtypes = [C.c_int, P, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class OracleResult:
+    status: int
+    x: np.ndarray
+    y: np.ndarray
+    reduced_costs: np.ndarray
+    objective: float
+    iterations: int
+    restarts: int
+    rejected_steps: int
+    step_size: float
+    primal_weight: float
+    primal_res: float
+    dual_res: float
+    gap: float
+    scale_obj: float
+    history: list = field(default_factory=list)
+
+
+def power_start(n: int) -> np.ndarray:
+    """``np.random.default_rng(0).standard_normal(n)`` (pdlp.py:142-143)."""
+    return np.random.default_rng(0).standard_normal(n)
+
+
+def solve(prob, eps_abs=1e-8, eps_rel=1e-8, feas_cap=1e-6, max_iter=200_000, time_limit=600.0,
+          check_every=64, stepsize_every=8, threads=0, hist_cap=100_000) -> OracleResult:
+    A = prob.A
+    m, n = A.m, A.n
+    par = _Params(eps_abs, eps_rel, feas_cap, max_iter, time_limit, check_every, stepsize_every,
+                  0.2, 0.8, 0.36, 0.5)
+    info = _Info()
+    x = np.zeros(n)
+    y = np.zeros(m)
+    d = np.zeros(n)
+    hist = np.zeros((hist_cap, 5))
+    arrs = [np.ascontiguousarray(a) for a in (A.cp, A.ci, A.cx, A.rp, A.ri, A.rx, prob.c,
+                                              prob.row_lb, prob.row_ub, prob.col_lb, prob.col_ub)]
+    v0 = power_start(n)
+    rc = lib().oracle_pdlp_solve(m, n, *[_p(a) for a in arrs], float(prob.obj_offset),
+                                 int(int(prob.sense) == -1), _p(v0), C.byref(par), _p(x), _p(y),
+                                 _p(d), C.byref(info), _p(hist), hist_cap, threads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_pdlp_solve failed ({rc})")
+    h = min(info.history_len, hist_cap)
+    return OracleResult(info.status, x, y, d, info.objective, info.iterations, info.restarts,
+                        info.rejected_steps, info.step_size, info.primal_weight,
+                        info.primal_res, info.dual_res, info.gap, info.scale_obj,
+                        [tuple(r) for r in hist[:h].tolist()])
+
+
+def scale(prob, c=None):
+    """``(row, col, obj)`` factors of ``scale_problem(prob, "pdlp")``; ``c`` defaults
+    to ``prob.c`` (pass the negated costs for a maximisation, as solve_pdlp does)."""
+    A = prob.A
+    row = np.zeros(A.m)
+    col = np.zeros(A.n)
+    obj = C.c_double()
+    cc = np.ascontiguousarray(prob.c if c is None else c, dtype=np.float64)
+    arrs = [np.ascontiguousarray(a) for a in (A.cp, A.ci, A.cx, A.rp, A.ri, A.rx)]
+    rc = lib().oracle_scale(A.m, A.n, *[_p(a) for a in arrs], _p(cc), _p(row), _p(col),
+                            C.byref(obj), None)
+    if rc != 0:
+        raise RuntimeError("oracle_scale failed")
+    return row, col, obj.value
+
+
+def spmv(rp, ri, rx, v, rows):
+    out = np.zeros(rows)
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    ri = np.ascontiguousarray(ri, dtype=np.int32)
+    rx = np.ascontiguousarray(rx, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    lib().oracle_spmv(rows, _p(rp), _p(ri), _p(rx), _p(v), _p(out))
+    return out
