@@ -1,0 +1,235 @@
+/*
+ * sovlp.h -- C ABI of libsovlp.so, the MI355X (gfx950) PDLP engine.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero code otherwise;
+ *     sovlp_last_error() returns the message of the last failure on the
+ *     calling thread.  Codes: 1 invalid argument, 2 HIP runtime error,
+ *     4 unsupported, 5 self-test mismatch.
+ *   - Device pointers are plain pointers owned by the caller (PyTorch tensors
+ *     in the Python host layer).  The library allocates no device memory except
+ *     inside sovlp_selftest; engines carve the caller's workspace.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Launches are
+ *     asynchronous unless a function says it synchronises.
+ *   - Row/column pointer arrays of the engines are int32 offsets (an LP or a
+ *     batch member holds fewer than 2^31 nonzeros); the Backend surface keeps
+ *     the reference's int64 pointers.
+ *
+ * The reference is SOVOPT (src/sovopt); each entry names what it replaces.
+ */
+#ifndef SOVLP_H
+#define SOVLP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *sovlp_stream_t;
+
+/* ---- library --------------------------------------------------------- */
+
+/* Message of the last error on this thread ("" if none). */
+const char *sovlp_last_error(void);
+
+/* ABI version (major*100 + minor). */
+int sovlp_version(void);
+
+/* Known-answer test [[1,0,2],[0,3,0]] . [1,2,3] == [7,6] through the plain
+ * and the tiled SpMV kernels, FP64 and FP32; synchronises.
+ * Replaces gpu_selftest, src/sovopt/core/backend.py:499-535. */
+int sovlp_selftest(void);
+
+/* Device wall-clock rate in Hz (for converting engine tick counts). */
+int sovlp_clock_rate(int device, double *hz);
+
+/* ---- Backend surface: src/sovopt/core/backend.py:317-493 ------------- */
+
+/* y = A x, A in CSR (int64 pointers, int32 indices).  lanes_per_row in
+ * {1,2,4,8,16,32,64} lanes reduce one row (the reference uses one 32-lane
+ * warp per row, backend.py:118-152; 64 = one wavefront per row).
+ * Replaces Backend.spmv, backend.py:373-384. */
+int sovlp_spmv_csr_f64(int rows, const int64_t *rp, const int32_t *ri, const double *rx,
+                       const double *x, double *y, int lanes_per_row, sovlp_stream_t stream);
+int sovlp_spmv_csr_f32(int rows, const int64_t *rp, const int32_t *ri, const float *rx,
+                       const float *x, float *y, int lanes_per_row, sovlp_stream_t stream);
+
+/* xnew = clamp(x - tau (c + aty), lo, hi).  Replaces Backend.primal_step,
+ * backend.py:404-415 (kernel 204-215). */
+int sovlp_primal_step_f64(double *xnew, const double *x, const double *c, const double *aty,
+                          const double *lo, const double *hi, double tau, int n,
+                          sovlp_stream_t stream);
+/* ynew = w - sigma clamp(w/sigma, rl, ru), w = y + sigma ax.  Replaces
+ * Backend.dual_step, backend.py:417-434 (kernel 219-231). */
+int sovlp_dual_step_f64(double *ynew, const double *y, const double *ax, const double *rl,
+                        const double *ru, double sigma, int m, sovlp_stream_t stream);
+/* In place x <- clamp(x - tau (c + aty)), dx = xnew - xold.  Replaces
+ * Backend.pdhg_primal, backend.py:441-452 (kernel 241-255). */
+int sovlp_pdhg_primal_f64(double *x, double *dx, const double *c, const double *aty,
+                          const double *lo, const double *hi, double tau, int n,
+                          sovlp_stream_t stream);
+/* ax += adx; y <- prox(y + sigma (ax + adx)); dy = ynew - yold.  Replaces
+ * Backend.pdhg_dual, backend.py:454-467 (kernel 260-279). */
+int sovlp_pdhg_dual_f64(double *y, double *dy, double *ax, const double *adx, const double *rl,
+                        const double *ru, double sigma, int m, sovlp_stream_t stream);
+/* acc += inc; sum += eta v.  Replaces Backend.pdhg_accum, backend.py:469-479
+ * (kernel 282-291). */
+int sovlp_pdhg_accum_f64(double *acc, const double *inc, double *sum, const double *v,
+                         double eta, int n, sovlp_stream_t stream);
+
+/* FP64 reductions into a device scalar, fixed summation order (bitwise
+ * repeatable).  Replace Backend.dot / nrm2 / nrm_inf, backend.py:483-490. */
+int sovlp_dot_f64(const double *a, const double *b, int64_t n, double *out, sovlp_stream_t stream);
+int sovlp_nrm2_f64(const double *a, int64_t n, double *out, sovlp_stream_t stream);
+int sovlp_nrminf_f64(const double *a, int64_t n, double *out, sovlp_stream_t stream);
+/* FP32 vectors reduced in FP64. */
+int sovlp_dot_f32(const float *a, const float *b, int64_t n, double *out, sovlp_stream_t stream);
+int sovlp_nrm2_f32(const float *a, int64_t n, double *out, sovlp_stream_t stream);
+
+/* ---- tiled (CSR-Adaptive) SpMV: the engines' sparse product ---------- */
+
+/* Host function.  Cuts rows into tiles of consecutive rows with at most
+ * tile_nnz nonzeros and at most tile_rows rows; a row longer than tile_nnz is
+ * a tile of its own.  tiles_out needs rows+1 entries; *ntiles_out receives the
+ * tile count (tiles_out[0..ntiles] are row starts).  No reference counterpart
+ * (backend.py:118-152 is row-per-warp only). */
+int sovlp_plan_tiles(int rows, const int64_t *ptr, int tile_nnz, int tile_rows,
+                     int32_t *tiles_out, int32_t *ntiles_out);
+
+/* Host function.  Splits ntiles tiles into `parts` contiguous ranges of equal
+ * cost (nonzeros + rows); part_out needs parts+1 entries. */
+int sovlp_partition_tiles(int ntiles, const int32_t *tiles, const int64_t *ptr, int parts,
+                          int32_t *part_out);
+
+/* Tile geometry compiled into the engines. */
+int sovlp_tile_geometry(int precision, int *block, int *tile_nnz);
+
+/* y = A x through the engines' tile pass (ptr32 = int32 row pointers).
+ * `part` has grid+1 entries (sovlp_partition_tiles with parts = grid). */
+int sovlp_spmv_tiled_f64(int rows, const int32_t *ptr32, const int32_t *idx, const double *val,
+                         const int32_t *tiles, int ntiles, const int32_t *part, int grid,
+                         const double *x, double *y, sovlp_stream_t stream);
+int sovlp_spmv_tiled_f32(int rows, const int32_t *ptr32, const int32_t *idx, const float *val,
+                         const int32_t *tiles, int ntiles, const int32_t *part, int grid,
+                         const float *x, float *y, sovlp_stream_t stream);
+
+/* ---- PDLP engine: src/sovopt/lp/pdlp.py:238-494 ---------------------- */
+
+/* PDLPParams, pdlp.py:79-116 (same fields and defaults). */
+typedef struct {
+  double eps_abs, eps_rel, feas_cap;
+  int64_t max_iter;
+  double time_limit;
+  int32_t check_every, stepsize_every;
+  double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
+} sovlp_params;
+
+/* One LP in minimisation form (costs and offset already negated for a
+ * maximisation; `maximise` only restores the textbook signs on output).
+ * All pointers are device pointers.  Bounds use +-1e30 for infinity. */
+typedef struct {
+  int32_t m, n;
+  int64_t nnz;
+  const int32_t *rp, *ri;   /* CSR of A: rp has m+1 int32 offsets */
+  const double *rx;
+  const int32_t *cp, *ci;   /* CSC of A (= CSR of A^T): cp has n+1 entries */
+  const double *cx;
+  const double *c, *rl, *ru, *lo, *hi;
+  const double *v0;         /* power-iteration start: default_rng(0).standard_normal(n) */
+  const int32_t *tiles_a;   /* sovlp_plan_tiles of A with the engine's geometry */
+  int32_t ntiles_a;
+  const int32_t *tiles_t;   /* ... of A^T */
+  int32_t ntiles_t;
+  double obj_offset;
+  int32_t maximise;
+} sovlp_lp;
+
+/* PDLPInfo, pdlp.py:119-132, plus engine bookkeeping. */
+typedef struct {
+  int32_t status;           /* Status codes, src/sovopt/core/problem.py:58-69; -1 running */
+  int32_t stage;            /* 0 setup pending, 1 iterating, 2 report pending, 3 done */
+  int64_t iterations, restarts, rejected_steps, history_len;
+  double primal_res, dual_res, gap, primal_obj, dual_obj;
+  double step_size, primal_weight, objective, scale_obj;
+  int64_t device_ticks;     /* device clock ticks spent (sovlp_clock_rate) */
+} sovlp_info;
+
+/* Workspace bytes for one LP at `precision` 64 (FP64 iterates) or 32 (FP32
+ * iterates and matrix, FP64 reductions and termination test), with room for
+ * `history_cap` history rows. */
+int sovlp_pdlp_workspace_bytes(const sovlp_lp *lp, int precision, int64_t history_cap,
+                               int64_t *bytes);
+
+/* Opaque engine handle (host memory; freed by sovlp_pdlp_destroy). */
+typedef struct sovlp_pdlp sovlp_pdlp;
+
+/* Prepares a solve in the caller's workspace (>= workspace_bytes) and records
+ * 64 phases of it into a hipGraph.  `team` 0 = choose, 1 = one workgroup per
+ * phase, 2 = W workgroups per phase (W from the tile count) with a one-
+ * workgroup reduction launch after each phase.  Synchronises. */
+int sovlp_pdlp_create(const sovlp_lp *lp, const sovlp_params *params, int precision,
+                      void *workspace, int64_t workspace_bytes, int64_t history_cap, int team,
+                      sovlp_stream_t stream, sovlp_pdlp **out);
+
+/* Replays the graph until the solve has advanced `iteration_budget`
+ * iterations or terminated.  `seconds_left` is the device time budget of the
+ * solve; it is taken on the first call.  Synchronises. */
+int sovlp_pdlp_advance(sovlp_pdlp *h, int64_t iteration_budget, double seconds_left,
+                       sovlp_stream_t stream);
+
+/* Copies the engine state to *info; synchronises the stream. */
+int sovlp_pdlp_info(sovlp_pdlp *h, sovlp_info *info, sovlp_stream_t stream);
+
+/* Device pointers of the report (x, y, reduced costs, FP64, original units)
+ * and of the history rows (k, pres, dres, gap, pobj), valid once stage == 3. */
+int sovlp_pdlp_outputs(sovlp_pdlp *h, double **x, double **y, double **d, double **history);
+
+/* Workgroups per phase chosen by sovlp_pdlp_create. */
+int sovlp_pdlp_width(sovlp_pdlp *h, int *workgroups);
+
+int sovlp_pdlp_destroy(sovlp_pdlp *h);
+
+/* ---- batched PDLP: many independent LPs in one persistent launch ------
+ * No reference counterpart: the reference solves one LP per call
+ * (pdlp.py:238); its only batched PDHG shares one A across B&B nodes
+ * (src/sovopt/mip/bnr.py:176-301).  Here each LP keeps its own matrix,
+ * scaling, step size, restarts and termination; one workgroup solves one LP
+ * at a time and pulls the next from a device queue (largest first), so the
+ * arithmetic of an LP does not depend on the other LPs in the batch.
+ * Tile plans must use sovlp_batch_geometry's tile size and row count. */
+typedef struct sovlp_batch sovlp_batch;
+
+/* Tile geometry (workgroup size and tile nonzeros) of the batched engine
+ * for `precision` and `block` (64 or 256 threads per LP). */
+int sovlp_batch_geometry(int precision, int block, int *tile_rows, int *tile_nnz);
+
+/* Workspace for `count` LPs (host array of descriptors with device pointers). */
+int sovlp_batch_workspace_bytes(int count, const sovlp_lp *lps, int precision,
+                                int64_t history_cap, int64_t *bytes);
+
+/* Lays out the batch in the workspace and records 64 phase launches (one
+ * workgroup of `block` threads per LP each) into a hipGraph.  `block` is 64
+ * or 256.  Synchronises. */
+int sovlp_batch_create(int count, const sovlp_lp *lps, const sovlp_params *params,
+                       int precision, int block, void *workspace, int64_t workspace_bytes,
+                       int64_t history_cap, sovlp_stream_t stream, sovlp_batch **out);
+
+/* Solves every LP of the batch from scratch: resets the states, then replays
+ * the graph until every LP has terminated (or `max_graphs` > 0 replays).
+ * `seconds_per_lp` is each LP's device time budget.  Synchronises. */
+int sovlp_batch_run(sovlp_batch *h, double seconds_per_lp, int64_t max_graphs,
+                    sovlp_stream_t stream, int64_t *graphs_run);
+
+/* Copies the `count` engine states to infos[]; synchronises. */
+int sovlp_batch_infos(sovlp_batch *h, sovlp_info *infos, sovlp_stream_t stream);
+
+/* Device pointers of LP `index`'s report (FP64, original units). */
+int sovlp_batch_outputs(sovlp_batch *h, int index, double **x, double **y, double **d);
+
+int sovlp_batch_destroy(sovlp_batch *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOVLP_H */

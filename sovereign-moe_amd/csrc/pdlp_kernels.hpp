@@ -1,0 +1,141 @@
+// pdlp_kernels.hpp -- the PDLP launch kernels (device code only).
+//
+// Each launch runs one phase (pdlp_solver.hpp) of each LP it serves:
+//   pdlp_step_kernel     one workgroup per LP; reduces and takes the transition
+//   pdlp_phase_kernel    W workgroups share one LP's phase, write partial sums
+//   pdlp_decide_kernel   one workgroup reduces the W partials (fixed order) and
+//                        takes the transition
+//   state_reset_kernel   puts a batch back at the start of its solves
+#pragma once
+#include "pdlp_solver.hpp"
+
+namespace sov {
+
+struct StepCtl {             // device-resident control block (read by every launch)
+  Params P;
+  long long ticks;           // time budget in device clock ticks, taken when a solve starts
+};
+
+__device__ __forceinline__ void init_part(int ph, double (&part)[kMaxRed]) {
+#pragma unroll
+  for (int k = 0; k < kMaxRed; ++k) part[k] = 0.0;
+  if (ph == PH_PC_APPLY) part[1] = -__builtin_inf();
+}
+
+// Workgroup totals of a phase's partial sums: CHECK has NCHK values, every
+// other phase at most 6.  Leaves the totals in red[(BLOCK/64)*kMaxRed + k].
+template <int BLOCK>
+__device__ __forceinline__ void reduce_phase_to_lds(int ph, const double (&part)[kMaxRed],
+                                                    double* red) {
+  const uint32_t mm = phase_maxmask(ph);
+  if (ph == PH_CHECK) {
+    block_reduce_to_lds<BLOCK, kMaxRed>(part, mm, red);
+  } else {
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = part[k];
+    block_reduce_to_lds<BLOCK, 6>(v, mm, red);
+  }
+}
+
+// Thread 0: apply the transition of phase `ph` (totals `t` in LDS) to the
+// state and store it; count the LP as done when it terminates.
+template <class T>
+__device__ __forceinline__ void take_transition(const LpDev<T>& L, const LpState& s, int ph,
+                                                const double* t, const StepCtl* ctl,
+                                                unsigned* done, LpState& ss) {
+  ss = s;
+  if (ph == PH_INIT) {
+    ss.tick0 = (long long)wall_clock64();
+    ss.ticks_left = ctl->ticks;
+  }
+  Transition tr{ctl->P, L.offset0, L.flip};
+  ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap);
+  if (ss.phase == PH_DONE) {
+    ss.ticks = (long long)wall_clock64() - ss.tick0;
+    atomicAdd(done, 1u);
+  }
+  *L.st = ss;
+}
+
+// One workgroup per LP: run the LP's current phase, reduce, take the transition.
+template <class T, int BLOCK, int TNNZ>
+__global__ void __launch_bounds__(BLOCK)
+    pdlp_step_kernel(const LpDev<T>* __restrict__ lps, const StepCtl* __restrict__ ctl,
+                     unsigned* done) {
+  __shared__ __align__(16) double lds_raw[2 * TNNZ];
+  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
+  __shared__ LpState ss;
+  const LpDev<T>& L = lps[blockIdx.x];
+  const LpState s = *L.st;
+  const int ph = uniform(s.phase);
+  if (ph == PH_DONE) return;
+  double part[kMaxRed];
+  init_part(ph, part);
+  PhaseWork<T, BLOCK, TNNZ> W(L, s, reinterpret_cast<T*>(lds_raw), 0, 1);
+  W.run(ph, part);
+  reduce_phase_to_lds<BLOCK>(ph, part, red);
+  if (threadIdx.x == 0) take_transition(L, s, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
+}
+
+// W workgroups share one LP's phase; each writes its partial sums.
+template <class T, int BLOCK, int TNNZ>
+__global__ void __launch_bounds__(BLOCK) pdlp_phase_kernel(const LpDev<T>* __restrict__ lp) {
+  __shared__ __align__(16) double lds_raw[2 * TNNZ];
+  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
+  const LpDev<T>& L = *lp;
+  const LpState s = *L.st;
+  const int ph = uniform(s.phase);
+  if (ph == PH_DONE) return;
+  double part[kMaxRed];
+  init_part(ph, part);
+  PhaseWork<T, BLOCK, TNNZ> W(L, s, reinterpret_cast<T*>(lds_raw), (int)blockIdx.x,
+                              (int)gridDim.x);
+  W.run(ph, part);
+  reduce_phase_to_lds<BLOCK>(ph, part, red);
+  constexpr int NW = BLOCK / kWave;
+  if (threadIdx.x < kMaxRed)
+    L.partials[(size_t)blockIdx.x * kMaxRed + threadIdx.x] = red[NW * kMaxRed + threadIdx.x];
+}
+
+// One workgroup: reduce the W partials of the phase just run (fixed order) and
+// take the transition.
+template <class T, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+    pdlp_decide_kernel(const LpDev<T>* __restrict__ lp, int W, const StepCtl* __restrict__ ctl,
+                       unsigned* done) {
+  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
+  __shared__ LpState ss;
+  const LpDev<T>& L = *lp;
+  const LpState s = *L.st;
+  const int ph = uniform(s.phase);
+  if (ph == PH_DONE) return;
+  const uint32_t mm = phase_maxmask(ph);
+  double part[kMaxRed];
+#pragma unroll
+  for (int k = 0; k < kMaxRed; ++k) part[k] = ((mm >> k) & 1u) ? -__builtin_inf() : 0.0;
+  for (int b = threadIdx.x; b < W; b += BLOCK) {
+    const double* pb = L.partials + (size_t)b * kMaxRed;
+#pragma unroll
+    for (int k = 0; k < kMaxRed; ++k)
+      part[k] = ((mm >> k) & 1u) ? nanmax(part[k], pb[k]) : part[k] + pb[k];
+  }
+  block_reduce_to_lds<BLOCK, kMaxRed>(part, mm, red);
+  if (threadIdx.x == 0) take_transition(L, s, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
+}
+
+// Puts every LP of a batch back at the start of its solve (a kernel rather
+// than a memset, so it can sit in a replayed graph).
+template <class T>
+__global__ void state_reset_kernel(const LpDev<T>* __restrict__ lps, int count, unsigned* done) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *done = 0u;
+  if (i >= count) return;
+  LpState s{};
+  s.phase = PH_INIT;
+  s.status = ST_RUNNING;
+  *lps[i].st = s;
+}
+
+
+}  // namespace sov

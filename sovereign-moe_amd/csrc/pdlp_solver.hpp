@@ -1,0 +1,801 @@
+// pdlp_solver.hpp -- restarted-average PDHG (PDLP) on the device, as a state machine.
+//
+// The reference loop (src/sovopt/lp/pdlp.py:238-494; line numbers below are
+// to that file unless stated) is cut into PHASES.  A phase is the work between
+// two reductions: one tile pass over A or A^T with a fused epilogue, or one
+// elementwise sweep.  Every kernel launch executes exactly one phase of each LP
+// it serves, so no launch contains a loop over iterations, and the iteration
+// state lives in HBM (LpState) between launches:
+//
+//   batched LPs   one workgroup per LP per launch; the workgroup reduces its
+//                 partial sums in LDS and applies the transition itself.
+//   one large LP  W workgroups share a phase and write partial sums; a
+//                 one-workgroup "decide" launch reduces them in a fixed order
+//                 and applies the transition.  Kernel boundaries take the
+//                 place of grid barriers.
+//
+// Phases and the reference lines they restate:
+//   INIT, RUIZ_MAX, RUIZ_APPLY   Ruiz equilibration (numerics/scaling.py:149-170)
+//   PC_SUM, PC_APPLY             Pock-Chambolle + power-of-two rounding
+//                                (scaling.py:274-309, 119-131, 349-352) and the
+//                                objective scale (scaling.py:422-444)
+//   SCALE                        scaled copy, bounds (scaling.py:446-456), x0 =
+//                                clip(0, lo, hi), y0 = 0 (274-275), norms for
+//                                omega0 (292-296) and for _converged (523-528)
+//   AX0, POW_A, POW_T            ax = A x0 (279) and 20 power iterations for
+//                                ||A||_2 (140-159, 289-290)
+//   KKT0                         initial merit (304)
+//   P_TRIAL, A_TRIAL             adaptive step trial (337-381): primal step
+//                                (backend.py:204-215), A dx, dual step on
+//                                ax + 2 adx (backend.py:219-231), three dots
+//   B_ADAPT                      accepted trial: A^T dy, x,y <- x',y', ax += adx,
+//                                aty += ady, sums with the updated eta (383-389)
+//   A_FUSED, B_FUSED             fused iteration (390-399; backend.py:241-291);
+//                                B phases also form the NEXT iteration's primal
+//                                projection in their epilogue
+//   CHECK                        scaled KKT of current and average points
+//                                (191-225), unscaled KKT of both (497-520),
+//                                termination (523-530), restart test (405-441)
+//   RESTART                      restart to the candidate and primal weight
+//                                update (443-463)
+//   COPY_AVG                     converged on the average point (424-425)
+//   FINAL, FINAL_D               unscale and report (465-494)
+#pragma once
+#include "sov_device.hpp"
+
+namespace sov {
+
+struct Params {              // same layout as sovlp_params in include/sovlp.h
+  double eps_abs, eps_rel, feas_cap;
+  long long max_iter;
+  double time_limit;
+  int check_every, stepsize_every;
+  double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
+};
+
+enum Phase : int {
+  PH_INIT = 0, PH_RUIZ_MAX, PH_RUIZ_APPLY, PH_PC_SUM, PH_PC_APPLY, PH_SCALE, PH_AX0, PH_POW_A,
+  PH_POW_T, PH_KKT0, PH_P_TRIAL, PH_A_TRIAL, PH_B_ADAPT, PH_A_FUSED, PH_B_FUSED, PH_CHECK,
+  PH_RESTART, PH_COPY_AVG, PH_FINAL, PH_FINAL_D, PH_DONE
+};
+
+enum NextMode : int { NEXT_NONE = 0, NEXT_FUSED = 1, NEXT_TRIAL = 2 };
+
+struct LpState {             // everything the loop carries between phases
+  double eta, omega, sum_eta, mu_restart, mu_last, mu_cand, anorm, sobj, ptol, dtol;
+  double dev, nv, nrm, est, dxdx;
+  double fin[5];             // unscaled (pres, dres, gap, pobj, dobj) at OPTIMAL
+  double objective;
+  long long k, epoch, attempts, restarts, rejected, hist_len;
+  long long tick0;           // device clock when the solve started (phase INIT)
+  long long ticks;           // device clock ticks used (set when the solve ends)
+  long long ticks_left;      // time budget in ticks
+  int phase, status, sweep, pit, trial, primal_ready, time_flag, use_avg, next_mode, pad;
+};
+
+template <class T>
+struct LpDev {
+  int m, n, ntA, ntT, flip, pad;
+  double offset0;            // objective offset of the minimisation form
+  // original problem (minimisation form), FP64
+  const int32_t *ap, *ai;
+  const double* av;          // CSR of A
+  const int32_t *tp, *ti;
+  const double* tv;          // CSC of A == CSR of A^T
+  const double *c0, *lo0, *hi0, *rl0, *ru0, *v0;
+  const int32_t *tilesA, *tilesT, *partA, *partT;
+  // scaled problem
+  T *sav, *stv;
+  double *rowf, *colf;
+  T *c, *lo, *hi, *rl, *ru;
+  // iterates (columns)
+  T *x, *xn, *dx, *aty, *sumx, *xr, *atyf, *atya;
+  // iterates (rows)
+  T *y, *yn, *dy, *ax, *adx, *sumy, *yr, *axf, *axa;
+  double *tmpm, *tmpn;
+  // outputs
+  double *xo, *yo, *dout;
+  double* hist;
+  long long hist_cap;
+  LpState* st;
+  double* partials;          // [W][kMaxRed] per-workgroup partial sums (W-way phases)
+};
+
+__device__ __forceinline__ double pow2_round(double s) {
+  if (!(s > 0.0 && s <= 1.79769313486231570e308)) return 1.0;   // non-finite or <= 0
+  return exp2(rint(log2(s)));
+}
+
+template <class T>
+__device__ __forceinline__ T clampT(T v, T l, T u) {
+  return v < l ? l : (v > u ? u : v);
+}
+
+__device__ __forceinline__ double bsel_row(double y, double rl, double ru) {
+  return y > 0.0 ? (ru < kInf ? ru : 0.0) : (rl > -kInf ? rl : 0.0);
+}
+__device__ __forceinline__ double bsel_col(double d, double lo, double hi) {
+  return d > 0.0 ? (lo > -kInf ? lo : 0.0) : (hi < kInf ? hi : 0.0);
+}
+__device__ __forceinline__ double dres_of(double d, double lo, double hi) {
+  return (lo > -kInf ? 0.0 : fmax(d, 0.0)) + (hi < kInf ? 0.0 : fmax(-d, 0.0));
+}
+
+// Reduction slots of the CHECK phase
+enum { PC = 0, DC, CXC, SUC, BNC, PA, DA, CXA, SUA, BNA, PUC, DUC, CXUC, SUUC, BNUC,
+       PUA, DUA, CXUA, SUUA, BNUA, TFLAG, NCHK };
+
+// Which reduced values of a phase are maxima (the rest are sums)
+__host__ __device__ constexpr uint32_t phase_maxmask(int ph) {
+  return ph == PH_RUIZ_MAX ? 1u : ph == PH_PC_APPLY ? 3u : ph == PH_SCALE ? ((1u << 3) | (1u << 4))
+       : ph == PH_KKT0 ? (1u | 2u | 32u)
+       : ph == PH_CHECK ? ((1u << PC) | (1u << DC) | (1u << PA) | (1u << DA) | (1u << PUC) |
+                           (1u << DUC) | (1u << PUA) | (1u << DUA) | (1u << TFLAG))
+       : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Phase work.  `rank`/`size` split the LP's rows, columns and tiles among the
+// workgroups that share the phase (size == 1 for batched LPs).  Each call adds
+// this workgroup's contribution to part[]; the caller reduces.
+// ---------------------------------------------------------------------------
+template <class T, int BLOCK, int TNNZ>
+struct PhaseWork {
+  const LpDev<T>& l;
+  const LpState& s;
+  T* lds;
+  double* ldsd;
+  int rank, size;
+
+  __device__ PhaseWork(const LpDev<T>& l_, const LpState& s_, T* lds_, int rank_, int size_)
+      : l(l_), s(s_), lds(lds_), ldsd(reinterpret_cast<double*>(lds_)), rank(rank_),
+        size(size_) {}
+
+  __device__ __forceinline__ void erange(long long len, long long* a, long long* b) const {
+    *a = len * rank / size;
+    *b = len * (rank + 1) / size;
+  }
+  __device__ __forceinline__ void trange(int ntiles, const int32_t* part, int* a, int* b) const {
+    if (size == 1) {
+      *a = 0;
+      *b = ntiles;
+    } else {
+      *a = part[rank];
+      *b = part[rank + 1];
+    }
+  }
+
+  // next iteration's primal projection, in place (fused) or out of place (trial)
+  __device__ __forceinline__ void next_primal(int j, T xj, T aty, int nm, T tt, double& dxdx) {
+    if (nm == NEXT_NONE) return;
+    const T v = clampT<T>(xj - tt * (l.c[j] + aty), l.lo[j], l.hi[j]);
+    const T d = v - xj;
+    l.dx[j] = d;
+    if (nm == NEXT_FUSED) {
+      l.x[j] = v;
+    } else {
+      l.xn[j] = v;
+      dxdx += (double)d * (double)d;
+    }
+  }
+
+  // rank 0's thread 0 reports whether the time budget is spent
+  __device__ __forceinline__ double clock_flag() const {
+    if (rank == 0 && threadIdx.x == 0)
+      return ((long long)wall_clock64() - s.tick0) > s.ticks_left ? 1.0 : 0.0;
+    return 0.0;
+  }
+
+  __device__ __forceinline__ void run(int ph, double (&part)[kMaxRed]) {
+    long long i0, i1, j0, j1;
+    erange(l.m, &i0, &i1);
+    erange(l.n, &j0, &j1);
+    int a0, a1, b0, b1;
+    trange(l.ntA, l.partA, &a0, &a1);
+    trange(l.ntT, l.partT, &b0, &b1);
+    const int tid = threadIdx.x;
+
+    switch (ph) {
+      case PH_INIT: {
+        for (long long i = i0 + tid; i < i1; i += BLOCK) l.rowf[i] = 1.0;
+        for (long long j = j0 + tid; j < j1; j += BLOCK) l.colf[j] = 1.0;
+      } break;
+
+      case PH_RUIZ_MAX: {   // row/col max of |a_ij| r_i c_j; deviation from 1
+        auto prodA = [&](int q, int j, double (&v)[1]) { v[0] = fabs(l.av[q]) * l.colf[j]; };
+        auto epiA = [&](int i, const double (&sm)[1]) {
+          double r = l.rowf[i] * sm[0];
+          if (r <= 0.0) r = 1.0;
+          part[0] = nanmax(part[0], fabs(1.0 - r));
+          l.tmpm[i] = r;
+        };
+        tile_pass<BLOCK, TNNZ, 1, true, double>(l.ap, l.ai, l.tilesA, a0, a1, prodA, epiA, ldsd);
+        auto prodT = [&](int q, int i, double (&v)[1]) { v[0] = fabs(l.tv[q]) * l.rowf[i]; };
+        auto epiT = [&](int j, const double (&sm)[1]) {
+          double c = l.colf[j] * sm[0];
+          if (c <= 0.0) c = 1.0;
+          part[0] = nanmax(part[0], fabs(1.0 - c));
+          l.tmpn[j] = c;
+        };
+        tile_pass<BLOCK, TNNZ, 1, true, double>(l.tp, l.ti, l.tilesT, b0, b1, prodT, epiT, ldsd);
+      } break;
+
+      case PH_RUIZ_APPLY: {
+        for (long long i = i0 + tid; i < i1; i += BLOCK) l.rowf[i] *= 1.0 / sqrt(l.tmpm[i]);
+        for (long long j = j0 + tid; j < j1; j += BLOCK) l.colf[j] *= 1.0 / sqrt(l.tmpn[j]);
+      } break;
+
+      case PH_PC_SUM: {     // Pock-Chambolle sums (alpha = 1) on the Ruiz-scaled matrix
+        auto prodA = [&](int q, int j, double (&v)[1]) { v[0] = fabs(l.av[q]) * l.colf[j]; };
+        auto epiA = [&](int i, const double (&sm)[1]) {
+          const double r = l.rowf[i] * sm[0];
+          l.tmpm[i] = r <= 0.0 ? 1.0 : r;
+        };
+        tile_pass<BLOCK, TNNZ, 1, false, double>(l.ap, l.ai, l.tilesA, a0, a1, prodA, epiA, ldsd);
+        auto prodT = [&](int q, int i, double (&v)[1]) { v[0] = fabs(l.tv[q]) * l.rowf[i]; };
+        auto epiT = [&](int j, const double (&sm)[1]) {
+          const double c = l.colf[j] * sm[0];
+          l.tmpn[j] = c <= 0.0 ? 1.0 : c;
+        };
+        tile_pass<BLOCK, TNNZ, 1, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prodT, epiT, ldsd);
+      } break;
+
+      case PH_PC_APPLY: {   // power-of-two factors; extremes of |c_j col_j| over nonzeros
+        for (long long i = i0 + tid; i < i1; i += BLOCK)
+          l.rowf[i] = pow2_round(l.rowf[i] * (1.0 / sqrt(l.tmpm[i])));
+        for (long long j = j0 + tid; j < j1; j += BLOCK) {
+          const double cf = pow2_round(l.colf[j] * (1.0 / sqrt(l.tmpn[j])));
+          l.colf[j] = cf;
+          const double v = l.c0[j] * cf;
+          if (v != 0.0) {
+            part[0] = nanmax(part[0], fabs(v));
+            part[1] = nanmax(part[1], -fabs(v));
+          }
+        }
+      } break;
+
+      case PH_SCALE: {      // scaled copy, bounds, x0, y0; ||c'||^2 ||b'||^2 ||v0||^2 max|c| rhs
+        auto wA = [&](int q, int i, int j) { l.sav[q] = (T)(l.av[q] * (l.rowf[i] * l.colf[j])); };
+        tile_foreach<BLOCK, TNNZ>(l.ap, l.ai, l.tilesA, a0, a1, wA);
+        auto wT = [&](int q, int j, int i) { l.stv[q] = (T)(l.tv[q] * (l.rowf[i] * l.colf[j])); };
+        tile_foreach<BLOCK, TNNZ>(l.tp, l.ti, l.tilesT, b0, b1, wT);
+        const double so = s.sobj;
+        for (long long j = j0 + tid; j < j1; j += BLOCK) {
+          const double cf = l.colf[j];
+          const T cj = (T)((l.c0[j] * cf) * so);
+          const double lo0 = l.lo0[j], hi0 = l.hi0[j];
+          const T lo = (T)(lo0 <= -kInf ? -kInf : (lo0 >= kInf ? kInf : lo0 / cf));
+          const T hi = (T)(hi0 <= -kInf ? -kInf : (hi0 >= kInf ? kInf : hi0 / cf));
+          l.c[j] = cj;
+          l.lo[j] = lo;
+          l.hi[j] = hi;
+          const T x0 = clampT<T>(T(0), lo, hi);
+          l.x[j] = x0;
+          l.xr[j] = x0;
+          l.sumx[j] = T(0);
+          l.aty[j] = T(0);
+          part[0] += (double)cj * (double)cj;
+          part[2] += l.v0[j] * l.v0[j];
+          part[3] = nanmax(part[3], fabs(l.c0[j]));
+        }
+        for (long long i = i0 + tid; i < i1; i += BLOCK) {
+          const double rf = l.rowf[i];
+          const double rl0 = l.rl0[i], ru0 = l.ru0[i];
+          const T rl = (T)(rl0 <= -kInf ? -kInf : (rl0 >= kInf ? kInf : rl0 * rf));
+          const T ru = (T)(ru0 <= -kInf ? -kInf : (ru0 >= kInf ? kInf : ru0 * rf));
+          l.rl[i] = rl;
+          l.ru[i] = ru;
+          l.y[i] = T(0);
+          l.yr[i] = T(0);
+          l.sumy[i] = T(0);
+          const double b = (rl > (T)(-kInf) ? (double)rl : 0.0) + (ru < (T)kInf ? (double)ru : 0.0);
+          part[1] += b * b;
+          if (rl0 > -kInf) part[4] = nanmax(part[4], fabs(rl0));
+          if (ru0 < kInf) part[4] = nanmax(part[4], fabs(ru0));
+        }
+      } break;
+
+      case PH_AX0: {        // ax = A x0; v = v0 / ||v0|| (kept in xn)
+        auto prod = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * l.x[j]; };
+        auto epi = [&](int i, const T (&sm)[1]) { l.ax[i] = sm[0]; };
+        tile_pass<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, epi, lds);
+        const double nv = s.nv;
+        if (nv != 0.0)
+          for (long long j = j0 + tid; j < j1; j += BLOCK) l.xn[j] = (T)(l.v0[j] / nv);
+      } break;
+
+      case PH_POW_A: {      // yn = A v, v = xn (first) or dx / ||dx||
+        const bool first = s.pit == 0;
+        const double nrm = s.nrm;
+        auto prod = [&](int q, int j, T (&v)[1]) {
+          v[0] = l.sav[q] * (first ? l.xn[j] : (T)((double)l.dx[j] / nrm));
+        };
+        auto epi = [&](int i, const T (&sm)[1]) { l.yn[i] = sm[0]; };
+        tile_pass<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, epi, lds);
+      } break;
+
+      case PH_POW_T: {      // dx = A^T yn; ||dx||^2
+        auto prod = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * l.yn[i]; };
+        auto epi = [&](int j, const T (&sm)[1]) {
+          l.dx[j] = sm[0];
+          part[0] += (double)sm[0] * (double)sm[0];
+        };
+        tile_pass<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, epi, lds);
+      } break;
+
+      case PH_KKT0: {       // scaled KKT of (x0, y0, A x0, 0): pres dres c.x supp bnd time
+        for (long long i = i0 + tid; i < i1; i += BLOCK) {
+          const double a = l.ax[i], rl = l.rl[i], ru = l.ru[i], y = l.y[i];
+          part[0] = nanmax(part[0], fabs(fmax(rl - a, 0.0) + fmax(a - ru, 0.0)));
+          part[3] += bsel_row(y, rl, ru) * y;
+        }
+        for (long long j = j0 + tid; j < j1; j += BLOCK) {
+          const double c = l.c[j], d = c + (double)l.aty[j], lo = l.lo[j], hi = l.hi[j];
+          part[1] = nanmax(part[1], fabs(dres_of(d, lo, hi)));
+          part[2] += c * (double)l.x[j];
+          part[4] += bsel_col(d, lo, hi) * d;
+        }
+        part[5] = clock_flag();
+      } break;
+
+      case PH_P_TRIAL: {    // x' = clip(x - tau (c + aty)); dx = x' - x
+        const T tt = (T)(s.eta / s.omega);
+        for (long long j = j0 + tid; j < j1; j += BLOCK) {
+          const T xj = l.x[j];
+          const T v = clampT<T>(xj - tt * (l.c[j] + l.aty[j]), l.lo[j], l.hi[j]);
+          l.xn[j] = v;
+          const T d = v - xj;
+          l.dx[j] = d;
+          part[0] += (double)d * (double)d;
+        }
+      } break;
+
+      case PH_A_TRIAL: {    // adx = A dx; y' = prox(y + sigma (ax + 2 adx)); dy.adx, dy.dy
+        const T sg = (T)(s.eta * s.omega);
+        auto prod = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * l.dx[j]; };
+        auto epi = [&](int i, const T (&sm)[1]) {
+          const T a = sm[0];
+          l.adx[i] = a;
+          const T yo = l.y[i];
+          const T w = yo + sg * (l.ax[i] + T(2) * a);
+          const T pr = clampT<T>(w / sg, l.rl[i], l.ru[i]);
+          const T yv = w - sg * pr;
+          l.yn[i] = yv;
+          const T d = yv - yo;
+          l.dy[i] = d;
+          part[0] += (double)d * (double)a;
+          part[1] += (double)d * (double)d;
+        };
+        tile_pass<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, epi, lds);
+      } break;
+
+      case PH_B_ADAPT: {    // accepted trial
+        const T et = (T)s.eta, tt = (T)(s.eta / s.omega);
+        const int nm = s.next_mode;
+        double dxdx = 0.0;
+        auto prod = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * l.dy[i]; };
+        auto epi = [&](int j, const T (&sm)[1]) {
+          const T xj = l.xn[j];
+          l.x[j] = xj;
+          const T aty = l.aty[j] + sm[0];
+          l.aty[j] = aty;
+          l.sumx[j] += et * xj;
+          next_primal(j, xj, aty, nm, tt, dxdx);
+        };
+        tile_pass<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, epi, lds);
+        for (long long i = i0 + tid; i < i1; i += BLOCK) {
+          const T yv = l.yn[i];
+          l.y[i] = yv;
+          l.ax[i] += l.adx[i];
+          l.sumy[i] += et * yv;
+        }
+        part[0] += dxdx;
+      } break;
+
+      case PH_A_FUSED: {    // fused dual step (backend.py:260-279) and sum_y += eta y
+        const T sg = (T)(s.eta * s.omega), et = (T)s.eta;
+        auto prod = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * l.dx[j]; };
+        auto epi = [&](int i, const T (&sm)[1]) {
+          const T a = sm[0];
+          const T axn = l.ax[i] + a;
+          l.ax[i] = axn;
+          const T yo = l.y[i];
+          const T w = yo + sg * (axn + a);
+          const T pr = clampT<T>(w / sg, l.rl[i], l.ru[i]);
+          const T yv = w - sg * pr;
+          l.dy[i] = yv - yo;
+          l.y[i] = yv;
+          l.sumy[i] += et * yv;
+        };
+        tile_pass<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, epi, lds);
+      } break;
+
+      case PH_B_FUSED: {    // aty += A^T dy; sum_x += eta x (backend.py:282-291); next primal
+        const T et = (T)s.eta, tt = (T)(s.eta / s.omega);
+        const int nm = s.next_mode;
+        double dxdx = 0.0;
+        auto prod = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * l.dy[i]; };
+        auto epi = [&](int j, const T (&sm)[1]) {
+          const T aty = l.aty[j] + sm[0];
+          l.aty[j] = aty;
+          const T xj = l.x[j];
+          l.sumx[j] += et * xj;
+          next_primal(j, xj, aty, nm, tt, dxdx);
+        };
+        tile_pass<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, epi, lds);
+        part[0] += dxdx;
+      } break;
+
+      case PH_CHECK: check(part, a0, a1, b0, b1); break;
+
+      case PH_RESTART: {    // x, y <- candidate; ax, aty <- fresh products; distances
+        const bool ua = s.use_avg;
+        const double se = s.sum_eta;
+        for (long long j = j0 + tid; j < j1; j += BLOCK) {
+          const T xv = ua ? (T)((double)l.sumx[j] / se) : l.x[j];
+          l.x[j] = xv;
+          l.aty[j] = ua ? l.atya[j] : l.atyf[j];
+          const double d = (double)xv - (double)l.xr[j];
+          part[0] += d * d;
+          l.xr[j] = xv;
+          l.sumx[j] = T(0);
+        }
+        for (long long i = i0 + tid; i < i1; i += BLOCK) {
+          const T yv = ua ? (T)((double)l.sumy[i] / se) : l.y[i];
+          l.y[i] = yv;
+          l.ax[i] = ua ? l.axa[i] : l.axf[i];
+          const double d = (double)yv - (double)l.yr[i];
+          part[1] += d * d;
+          l.yr[i] = yv;
+          l.sumy[i] = T(0);
+        }
+      } break;
+
+      case PH_COPY_AVG: {
+        const double se = s.sum_eta;
+        for (long long j = j0 + tid; j < j1; j += BLOCK) l.x[j] = (T)((double)l.sumx[j] / se);
+        for (long long i = i0 + tid; i < i1; i += BLOCK) l.y[i] = (T)((double)l.sumy[i] / se);
+      } break;
+
+      case PH_FINAL: {      // x = clip(x col), y = -(y row / s); objective partial
+        for (long long j = j0 + tid; j < j1; j += BLOCK) {
+          double v = (double)l.x[j] * l.colf[j];
+          v = fmin(fmax(v, l.lo0[j]), l.hi0[j]);
+          l.xo[j] = v;
+          part[0] += l.c0[j] * v;
+        }
+        for (long long i = i0 + tid; i < i1; i += BLOCK) {
+          const double v = -(((double)l.y[i] * l.rowf[i]) / s.sobj);
+          l.yo[i] = l.flip ? -v : v;
+        }
+      } break;
+
+      case PH_FINAL_D: {    // reduced costs d = c - A^T y in original units
+        auto prod = [&](int q, int i, double (&p)[1]) { p[0] = l.tv[q] * l.yo[i]; };
+        auto epi = [&](int j, const double (&sm)[1]) {
+          const double c = l.flip ? -l.c0[j] : l.c0[j];
+          l.dout[j] = c - sm[0];
+        };
+        tile_pass<BLOCK, TNNZ, 1, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prod, epi, ldsd);
+      } break;
+
+      default: break;
+    }
+  }
+
+  // Two-vector passes give A x, A avg_x, A^T y, A^T avg_y; the epilogues
+  // accumulate the scaled KKT of the current point (with the maintained ax,
+  // aty, as _KKT.__call__ is given them) and of the average, and the unscaled
+  // KKT of both.  FP64 iterates: the scaled products are exact power-of-two
+  // multiples of the original ones (A' = R A C).  FP32 iterates: the unscaled
+  // KKT is recomputed in FP64 from the original matrix, as _unscaled_kkt does.
+  __device__ __forceinline__ void check(double (&v)[kMaxRed], int a0, int a1, int b0, int b1) {
+    const double se = s.sum_eta;
+    const double inv_se = se > 0.0 ? se : 1.0;
+    const double so = s.sobj;
+    constexpr bool kDerived = sizeof(T) == 8;
+
+    auto unscaled_row = [&](int i, double au, double aua, double y, double ya) {
+      const double rf = l.rowf[i], rl0 = l.rl0[i], ru0 = l.ru0[i];
+      const double yu = (y * rf) / so, yua = (ya * rf) / so;
+      v[PUC] = nanmax(v[PUC], fabs(fmax(rl0 - au, 0.0) + fmax(au - ru0, 0.0)));
+      v[SUUC] += bsel_row(yu, rl0, ru0) * yu;
+      v[PUA] = nanmax(v[PUA], fabs(fmax(rl0 - aua, 0.0) + fmax(aua - ru0, 0.0)));
+      v[SUUA] += bsel_row(yua, rl0, ru0) * yua;
+    };
+    auto unscaled_col = [&](int j, double atu, double atua, double x, double xa) {
+      const double cf = l.colf[j], lo0 = l.lo0[j], hi0 = l.hi0[j], c0 = l.c0[j];
+      const double xu = x * cf, xua = xa * cf;
+      const double du = c0 + atu, dua = c0 + atua;
+      v[PUC] = nanmax(v[PUC], fabs(fmax(lo0 - xu, 0.0) + fmax(xu - hi0, 0.0)));
+      v[DUC] = nanmax(v[DUC], fabs(dres_of(du, lo0, hi0)));
+      v[CXUC] += c0 * xu;
+      v[BNUC] += bsel_col(du, lo0, hi0) * du;
+      v[PUA] = nanmax(v[PUA], fabs(fmax(lo0 - xua, 0.0) + fmax(xua - hi0, 0.0)));
+      v[DUA] = nanmax(v[DUA], fabs(dres_of(dua, lo0, hi0)));
+      v[CXUA] += c0 * xua;
+      v[BNUA] += bsel_col(dua, lo0, hi0) * dua;
+    };
+
+    auto prodA = [&](int q, int j, T (&p)[2]) {
+      const T a = l.sav[q];
+      p[0] = a * l.x[j];
+      p[1] = a * (T)((double)l.sumx[j] / inv_se);
+    };
+    auto epiA = [&](int i, const T (&sm)[2]) {
+      l.axf[i] = sm[0];
+      l.axa[i] = sm[1];
+      const double rl = l.rl[i], ru = l.ru[i], y = l.y[i];
+      const double ya = (T)((double)l.sumy[i] / inv_se);
+      const double am = l.ax[i], ag = sm[1];
+      v[PC] = nanmax(v[PC], fabs(fmax(rl - am, 0.0) + fmax(am - ru, 0.0)));
+      v[SUC] += bsel_row(y, rl, ru) * y;
+      v[PA] = nanmax(v[PA], fabs(fmax(rl - ag, 0.0) + fmax(ag - ru, 0.0)));
+      v[SUA] += bsel_row(ya, rl, ru) * ya;
+      if (kDerived) {
+        const double rf = l.rowf[i];
+        unscaled_row(i, (double)sm[0] / rf, (double)sm[1] / rf, y, ya);
+      }
+    };
+    tile_pass<BLOCK, TNNZ, 2, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prodA, epiA, lds);
+
+    auto prodT = [&](int q, int i, T (&p)[2]) {
+      const T a = l.stv[q];
+      p[0] = a * l.y[i];
+      p[1] = a * (T)((double)l.sumy[i] / inv_se);
+    };
+    auto epiT = [&](int j, const T (&sm)[2]) {
+      l.atyf[j] = sm[0];
+      l.atya[j] = sm[1];
+      const double c = l.c[j], lo = l.lo[j], hi = l.hi[j], x = l.x[j];
+      const double xa = (T)((double)l.sumx[j] / inv_se);
+      const double d = c + (double)l.aty[j], da = c + (double)sm[1];
+      v[DC] = nanmax(v[DC], fabs(dres_of(d, lo, hi)));
+      v[CXC] += c * x;
+      v[BNC] += bsel_col(d, lo, hi) * d;
+      v[DA] = nanmax(v[DA], fabs(dres_of(da, lo, hi)));
+      v[CXA] += c * xa;
+      v[BNA] += bsel_col(da, lo, hi) * da;
+      if (kDerived) {
+        const double kf = l.colf[j] * so;
+        unscaled_col(j, (double)sm[0] / kf, (double)sm[1] / kf, x, xa);
+      }
+    };
+    tile_pass<BLOCK, TNNZ, 2, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prodT, epiT, lds);
+
+    if constexpr (!kDerived) {
+      auto prodA0 = [&](int q, int j, double (&p)[2]) {
+        const double a = l.av[q], cf = l.colf[j];
+        p[0] = a * ((double)l.x[j] * cf);
+        p[1] = a * ((double)(T)((double)l.sumx[j] / inv_se) * cf);
+      };
+      auto epiA0 = [&](int i, const double (&sm)[2]) {
+        const double y = l.y[i], ya = (T)((double)l.sumy[i] / inv_se);
+        unscaled_row(i, sm[0], sm[1], y, ya);
+      };
+      tile_pass<BLOCK, TNNZ, 2, false, double>(l.ap, l.ai, l.tilesA, a0, a1, prodA0, epiA0, ldsd);
+      auto prodT0 = [&](int q, int i, double (&p)[2]) {
+        const double a = l.tv[q], rf = l.rowf[i];
+        p[0] = a * (((double)l.y[i] * rf) / so);
+        p[1] = a * (((double)(T)((double)l.sumy[i] / inv_se) * rf) / so);
+      };
+      auto epiT0 = [&](int j, const double (&sm)[2]) {
+        const double x = l.x[j], xa = (T)((double)l.sumx[j] / inv_se);
+        unscaled_col(j, sm[0], sm[1], x, xa);
+      };
+      tile_pass<BLOCK, TNNZ, 2, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prodT0, epiT0, ldsd);
+    }
+    v[TFLAG] = clock_flag();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Transition: applies the reduced values of phase `ph` to the state and picks
+// the next phase.  Scalar logic only; runs on one thread.
+// ---------------------------------------------------------------------------
+struct Transition {
+  const Params& P;
+  double offset0;            // minimisation-form offset
+  int flip;                  // the caller's problem maximises
+
+  __device__ __forceinline__ double merit(const LpState& s, double p, double d, double cx,
+                                          double supp, double bnd, double w) const {
+    const double soff = offset0 * s.sobj;
+    const double pobj = cx + soff, dobj = (-supp + bnd) + soff;
+    const double g = fabs(pobj - dobj), a = w * p, b = d / w;
+    return sqrt(a * a + b * b + g * g);
+  }
+
+  __device__ __forceinline__ int next_mode(long long k1) const {
+    if (k1 % P.check_every == 0 || k1 >= P.max_iter) return NEXT_NONE;
+    return (k1 % P.stepsize_every == 0) ? NEXT_TRIAL : NEXT_FUSED;
+  }
+
+  // top of the loop body (pdlp.py:313-317, 335): limits, then the step flavour
+  __device__ __forceinline__ int loop_top(LpState& s) const {
+    if (s.k >= P.max_iter) {
+      s.status = ST_ITERATION_LIMIT;
+      return PH_FINAL;
+    }
+    if (s.k % P.check_every == 0 && s.time_flag) {
+      s.status = ST_TIME_LIMIT;
+      return PH_FINAL;
+    }
+    if (s.k % P.stepsize_every == 0) {
+      s.trial = 0;
+      ++s.attempts;
+      if (s.primal_ready) {
+        s.primal_ready = 0;
+        return PH_A_TRIAL;       // the previous B phase already formed x' and ||dx||^2
+      }
+      return PH_P_TRIAL;
+    }
+    s.primal_ready = 0;
+    s.next_mode = next_mode(s.k + 1);
+    return PH_A_FUSED;
+  }
+
+  // after a B phase: the iteration is complete (pdlp.py:401-403)
+  __device__ __forceinline__ int end_iteration(LpState& s, double dxdx) const {
+    s.sum_eta += s.eta;
+    ++s.k;
+    ++s.epoch;
+    s.primal_ready = s.next_mode != NEXT_NONE;
+    if (s.next_mode == NEXT_TRIAL) s.dxdx = dxdx;
+    if (s.k % P.check_every == 0) return PH_CHECK;
+    return loop_top(s);
+  }
+
+  // Called by one thread with `s` and `t` in LDS (out of line: the scalar logic
+  // then costs the phase kernels neither registers nor scratch).
+  __device__ __noinline__ int apply(LpState& s, int ph, const double* t, double* hist,
+                                    long long hist_cap) const {
+    switch (ph) {
+      case PH_INIT: s.sweep = 0; return PH_RUIZ_MAX;
+      case PH_RUIZ_MAX: s.dev = t[0]; return PH_RUIZ_APPLY;
+      case PH_RUIZ_APPLY:
+        if (s.dev < 1e-2 || s.sweep >= 9) return PH_PC_SUM;
+        ++s.sweep;
+        return PH_RUIZ_MAX;
+      case PH_PC_SUM: return PH_PC_APPLY;
+      case PH_PC_APPLY: {
+        s.sobj = 1.0;
+        if (t[0] > 0.0) {
+          const double g = exp2(rint(log2(sqrt(t[0] * (-t[1])))));
+          if (g > 0.0 && g <= 1.79769313486231570e308) s.sobj = 1.0 / g;
+        }
+        return PH_SCALE;
+      }
+      case PH_SCALE: {
+        s.ptol = fmin(P.eps_abs + P.eps_rel * t[4], P.feas_cap);
+        s.dtol = fmin(P.eps_abs + P.eps_rel * t[3], P.feas_cap);
+        const double cn = sqrt(t[0]), bn = sqrt(t[1]);
+        s.nv = sqrt(t[2]);
+        const double w = bn > 0.0 ? exp(log(fmax(cn, 1e-12)) - log(fmax(bn, 1e-12))) : 1.0;
+        s.omega = fmin(fmax(w, 1e-4), 1e4);
+        return PH_AX0;
+      }
+      case PH_AX0:
+        if (s.nv != 0.0) {
+          s.pit = 0;
+          s.est = 1.0;
+          return PH_POW_A;
+        }
+        s.anorm = 1.0;
+        s.eta = 1.0;
+        return PH_KKT0;
+      case PH_POW_A: return PH_POW_T;
+      case PH_POW_T: {
+        const double nrm = sqrt(t[0]);
+        if (!(nrm > 0.0)) {
+          s.anorm = 1.0;
+          s.eta = 1.0;
+          return PH_KKT0;
+        }
+        s.nrm = nrm;
+        s.est = sqrt(nrm);
+        ++s.pit;
+        if (s.pit < 20) return PH_POW_A;
+        s.anorm = fmax(s.est, 1e-12);
+        s.eta = 1.0 / s.anorm;
+        return PH_KKT0;
+      }
+      case PH_KKT0:
+        s.mu_restart = merit(s, t[0], t[1], t[2], t[3], t[4], s.omega);
+        s.mu_last = __builtin_inf();
+        s.sum_eta = 0.0;
+        s.k = s.epoch = s.attempts = s.restarts = s.rejected = s.hist_len = 0;
+        s.status = ST_RUNNING;
+        s.time_flag = t[5] > 0.0;
+        s.primal_ready = 0;
+        return loop_top(s);
+      case PH_P_TRIAL: s.dxdx = t[0]; return PH_A_TRIAL;
+      case PH_A_TRIAL: {
+        const double interaction = fabs(t[0]);
+        const double movement = 0.5 * (s.omega * s.dxdx + t[1] / s.omega);
+        bool accepted = false;
+        if (movement == 0.0) {
+          accepted = true;
+        } else {
+          const double lim = interaction > 0.0 ? movement / interaction : __builtin_inf();
+          const double tt = (double)(s.attempts + 1);
+          const double nxt =
+              fmax(fmin((1.0 - pow(tt, -0.3)) * lim, (1.0 + pow(tt, -0.6)) * s.eta), 1e-14);
+          if (s.eta <= lim) accepted = true;
+          else ++s.rejected;
+          s.eta = nxt;
+        }
+        if (accepted) {
+          s.next_mode = next_mode(s.k + 1);
+          return PH_B_ADAPT;
+        }
+        ++s.trial;
+        if (s.trial >= 60) {
+          s.status = ST_NUMERICAL;
+          return PH_FINAL;
+        }
+        ++s.attempts;
+        return PH_P_TRIAL;
+      }
+      case PH_B_ADAPT: return end_iteration(s, t[0]);
+      case PH_A_FUSED: return PH_B_FUSED;
+      case PH_B_FUSED: return end_iteration(s, t[0]);
+      case PH_CHECK: {
+        s.time_flag = t[TFLAG] > 0.0;
+        const bool have_avg = s.sum_eta > 0.0;
+        const double mu_c = merit(s, t[PC], t[DC], t[CXC], t[SUC], t[BNC], s.omega);
+        const double mu_a = have_avg ? merit(s, t[PA], t[DA], t[CXA], t[SUA], t[BNA], s.omega)
+                                     : __builtin_inf();
+        const bool use_avg = mu_a < mu_c;
+        const double mu_cand = use_avg ? mu_a : mu_c;
+        const double pu = use_avg ? t[PUA] : t[PUC], du = use_avg ? t[DUA] : t[DUC];
+        const double pou = (use_avg ? t[CXUA] : t[CXUC]) + offset0;
+        const double dou = (-(use_avg ? t[SUUA] : t[SUUC]) + (use_avg ? t[BNUA] : t[BNUC])) + offset0;
+        const double gu = fabs(pou - dou);
+        s.use_avg = use_avg;
+        s.primal_ready = 0;
+        if (pu <= s.ptol && du <= s.dtol && gu <= P.eps_abs + P.eps_rel * (fabs(pou) + fabs(dou))) {
+          s.status = ST_OPTIMAL;
+          s.fin[0] = pu; s.fin[1] = du; s.fin[2] = gu; s.fin[3] = pou; s.fin[4] = dou;
+          return use_avg ? PH_COPY_AVG : PH_FINAL;
+        }
+        if (s.hist_len < hist_cap) {
+          double* h = hist + 5 * s.hist_len;
+          h[0] = (double)s.k; h[1] = pu; h[2] = du; h[3] = gu; h[4] = pou;
+        }
+        ++s.hist_len;
+        const bool restart = mu_cand <= P.beta_sufficient * s.mu_restart ||
+                             (mu_cand <= P.beta_necessary * s.mu_restart && mu_cand > s.mu_last) ||
+                             (double)s.epoch >= P.beta_artificial * (double)(s.k > 1 ? s.k : 1);
+        s.mu_last = mu_cand;
+        s.mu_cand = mu_cand;
+        if (restart) return PH_RESTART;
+        return loop_top(s);
+      }
+      case PH_RESTART: {
+        const double ddx = sqrt(t[0]), ddy = sqrt(t[1]);
+        if (ddx > 1e-12 && ddy > 1e-12) {
+          const double th = P.primal_weight_smoothing;
+          const double w = exp(th * log(ddy / ddx) + (1.0 - th) * log(s.omega));
+          s.omega = fmin(fmax(w, 1e-6), 1e6);
+        }
+        s.mu_restart = s.mu_cand;
+        s.sum_eta = 0.0;
+        s.epoch = 0;
+        ++s.restarts;
+        return loop_top(s);
+      }
+      case PH_COPY_AVG: return PH_FINAL;
+      case PH_FINAL: {
+        // objective of the caller's problem (pdlp.py:473): prob.c = -c0 when maximising
+        const double cx = flip ? -t[0] : t[0];
+        s.objective = cx + (flip ? -offset0 : offset0);
+        return PH_FINAL_D;
+      }
+      case PH_FINAL_D: return PH_DONE;
+      default: return PH_DONE;
+    }
+  }
+};
+
+}  // namespace sov

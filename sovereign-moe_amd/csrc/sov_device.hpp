@@ -1,0 +1,208 @@
+// sov_device.hpp -- device primitives for the gfx950 PDLP engine.
+//
+//  * deterministic wave64 / workgroup reductions (fixed butterfly order, so a
+//    result never depends on timing);
+//  * the CSR-Adaptive tile pass: a sparse matrix is cut on the host into
+//    tiles of consecutive rows holding at most TNNZ nonzeros and at most
+//    BLOCK rows, or a single row longer than TNNZ.  A workgroup streams a
+//    tile's products (value * gathered vector entry) into LDS with fully
+//    coalesced loads, then reduces each row with 1..64 lanes chosen from the
+//    tile's row count, and hands every row sum to a fused epilogue.  Long
+//    rows are reduced by the whole workgroup.  The summation order is a
+//    function of the tile plan only, so products are bitwise repeatable.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sov {
+
+constexpr double kInf = 1.0e30;   // infinite-bound sentinel (tolerances.py:16)
+constexpr int kWave = 64;
+constexpr int kMaxRed = 24;       // values per team reduction
+
+enum : int {               // Status codes of src/sovopt/core/problem.py:58-69
+  ST_OPTIMAL = 0,
+  ST_ITERATION_LIMIT = 4,
+  ST_TIME_LIMIT = 5,
+  ST_NUMERICAL = 8,
+  ST_RUNNING = -1,
+};
+
+// A value every lane of the wave holds equally, moved to a scalar register so
+// that branches and loop bounds depending on it are scalar branches: barriers
+// inside them then never sit in an EXEC-masked region.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double nanmax(double a, double b) {
+  return (a != a) ? a : ((b != b) ? b : (a > b ? a : b));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T tmax(T a, T b) {
+  return a > b ? a : b;
+}
+
+// ------------------------------------------------------------------------
+// workgroup reduction of K doubles; bit k of maxmask selects max, else sum.
+// `red` is LDS with room for (BLOCK/64 + 2) * kMaxRed doubles.  Every thread
+// receives the totals.  Leaves the workgroup's totals in red[NW*kMaxRed + k].
+// ------------------------------------------------------------------------
+template <int BLOCK, int K>
+__device__ __forceinline__ void block_reduce_to_lds(const double (&v)[K], uint32_t maxmask,
+                                                    double* red) {
+  constexpr int NW = BLOCK / kWave;
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = ((maxmask >> k) & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
+    if (l == 0) red[w * kMaxRed + k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    const bool mx = (maxmask >> k) & 1u;
+    double s = red[k];
+    for (int q = 1; q < NW; ++q) {
+      double o = red[q * kMaxRed + k];
+      s = mx ? nanmax(s, o) : s + o;
+    }
+    red[NW * kMaxRed + k] = s;
+  }
+  __syncthreads();
+}
+
+template <int BLOCK, int K>
+__device__ __forceinline__ void block_reduce(double (&v)[K], uint32_t maxmask, double* red) {
+  constexpr int NW = BLOCK / kWave;
+  block_reduce_to_lds<BLOCK, K>(v, maxmask, red);
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = red[NW * kMaxRed + k];
+  __syncthreads();
+}
+
+// workgroup sum of NV values of type T (tile pass, long rows)
+template <int BLOCK, int NV, bool MAXOP, class T>
+__device__ __forceinline__ void block_combine(T (&s)[NV], T* scratch) {
+  constexpr int NW = BLOCK / kWave;
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    T v = s[k];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      T u = __shfl_xor(v, o);
+      v = MAXOP ? tmax(v, u) : v + u;
+    }
+    if (l == 0) scratch[k * NW + w] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    T v = scratch[k * NW];
+    for (int q = 1; q < NW; ++q) v = MAXOP ? tmax(v, scratch[k * NW + q]) : v + scratch[k * NW + q];
+    s[k] = v;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------
+// CSR-Adaptive tile pass.
+//   ptr   : row pointers (LP-local int32 offsets), rows+1 entries
+//   idx   : column index of each nonzero
+//   tiles : tile t covers rows [tiles[t], tiles[t+1])
+//   prod(q, j, v[NV]) writes the NV products of nonzero q (column j)
+//   epi(row, s[NV]) receives the NV row reductions (sum, or max if MAXOP)
+// ------------------------------------------------------------------------
+template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Prod, class Epi>
+__device__ __forceinline__ void tile_pass(const int32_t* __restrict__ ptr,
+                                          const int32_t* __restrict__ idx,
+                                          const int32_t* __restrict__ tiles, int t0, int t1,
+                                          Prod& prod, Epi& epi, T* lds) {
+  for (int t = uniform(t0); t < uniform(t1); ++t) {
+    const int r0 = uniform(tiles[t]), r1 = uniform(tiles[t + 1]);
+    const int p0 = uniform(ptr[r0]), p1 = uniform(ptr[r1]);
+    const int nz = p1 - p0, rows = r1 - r0;
+    if (nz > TNNZ) {
+      // one long row: the whole workgroup strides over it
+      T s[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) s[k] = T(0);
+      for (int q = p0 + (int)threadIdx.x; q < p1; q += BLOCK) {
+        T v[NV];
+        prod(q, idx[q], v);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) s[k] = MAXOP ? tmax(s[k], v[k]) : s[k] + v[k];
+      }
+      block_combine<BLOCK, NV, MAXOP, T>(s, lds);
+      if (threadIdx.x == 0) epi(r0, s);
+      __syncthreads();
+    } else {
+      for (int q = (int)threadIdx.x; q < nz; q += BLOCK) {
+        T v[NV];
+        prod(p0 + q, idx[p0 + q], v);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[k * TNNZ + q] = v[k];
+      }
+      __syncthreads();
+      int tpr = BLOCK / (rows > 0 ? rows : 1);
+      tpr = tpr >= 64 ? 64 : (tpr >= 32 ? 32 : (tpr >= 16 ? 16 : (tpr >= 8 ? 8 : (tpr >= 4 ? 4 : (tpr >= 2 ? 2 : 1)))));
+      const int grp = (int)threadIdx.x / tpr, ln = (int)threadIdx.x & (tpr - 1);
+      T s[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) s[k] = T(0);
+      const bool active = grp < rows;
+      if (active) {
+        const int a = ptr[r0 + grp] - p0, b = ptr[r0 + grp + 1] - p0;
+        for (int q = a + ln; q < b; q += tpr) {
+#pragma unroll
+          for (int k = 0; k < NV; ++k) s[k] = MAXOP ? tmax(s[k], lds[k * TNNZ + q]) : s[k] + lds[k * TNNZ + q];
+        }
+      }
+      for (int o = tpr >> 1; o > 0; o >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          T u = __shfl_xor(s[k], o);
+          s[k] = MAXOP ? tmax(s[k], u) : s[k] + u;
+        }
+      }
+      if (active && ln == 0) epi(r0 + grp, s);
+      __syncthreads();
+    }
+  }
+}
+
+// Visit every nonzero of a tile range with its row: fn(q, row, col).
+template <int BLOCK, int TNNZ, class Fn>
+__device__ __forceinline__ void tile_foreach(const int32_t* __restrict__ ptr,
+                                             const int32_t* __restrict__ idx,
+                                             const int32_t* __restrict__ tiles, int t0, int t1,
+                                             Fn& fn) {
+  for (int t = uniform(t0); t < uniform(t1); ++t) {
+    const int r0 = uniform(tiles[t]), r1 = uniform(tiles[t + 1]);
+    const int p0 = uniform(ptr[r0]), p1 = uniform(ptr[r1]);
+    for (int q = p0 + (int)threadIdx.x; q < p1; q += BLOCK) {
+      // binary search the row of q among [r0, r1)
+      int lo = r0, hi = r1 - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (ptr[mid] <= q) lo = mid; else hi = mid - 1;
+      }
+      fn(q, lo, idx[q]);
+    }
+  }
+}
+
+}  // namespace sov

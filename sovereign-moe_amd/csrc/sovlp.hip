@@ -1,0 +1,889 @@
+// sovlp.hip -- kernels and C ABI of libsovlp.so (gfx950).
+//
+// Kernels:
+//   spmv_vector_kernel   Backend surface SpMV, TPR lanes per row
+//   elementwise kernels  Backend surface steps (backend.py:204-291)
+//   reduce_kernel        Backend surface dot / nrm2 / nrm_inf, fixed order
+//   spmv_tiled_kernel    the engines' CSR-Adaptive tile pass as a bare SpMV
+//   pdlp_step_kernel     one PDLP phase of each LP, one workgroup per LP
+//   pdlp_phase_kernel    one PDLP phase of one LP over W workgroups
+//   pdlp_decide_kernel   reduces the W partial sums and takes the transition
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sovlp.h"
+#include "pdlp_kernels.hpp"
+
+using namespace sov;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(2, std::string(#expr) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+inline hipStream_t S(sovlp_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int kBlock = 256;
+constexpr int kTnnz64 = 2048;   // FP64 engines: 2 x 2048 x 8 B = 32 KiB LDS
+constexpr int kTnnz32 = 2048;   // FP32 engines: same tiles (the FP64 check passes share the LDS)
+
+// ---------------------------------------------------------------- Backend
+template <class T, int TPR>
+__global__ void spmv_vector_kernel(int rows, const int64_t* __restrict__ rp,
+                                   const int32_t* __restrict__ ri, const T* __restrict__ rx,
+                                   const T* __restrict__ x, T* __restrict__ y) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long row = g / TPR;
+  const int ln = (int)(g % TPR);
+  T s = T(0);
+  if (row < rows) {
+    const int64_t e = rp[row + 1];
+    for (int64_t p = rp[row] + ln; p < e; p += TPR) s += rx[p] * x[ri[p]];
+  }
+#pragma unroll
+  for (int o = TPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (row < rows && ln == 0) y[row] = s;
+}
+
+template <class T>
+int launch_spmv_vector(int rows, const int64_t* rp, const int32_t* ri, const T* rx, const T* x,
+                       T* y, int tpr, hipStream_t st) {
+  if (rows < 0) return fail(1, "rows < 0");
+  if (rows == 0) return 0;
+  const long long threads = (long long)rows * tpr;
+  const int blocks = (int)((threads + 255) / 256);
+  switch (tpr) {
+    case 1: spmv_vector_kernel<T, 1><<<blocks, 256, 0, st>>>(rows, rp, ri, rx, x, y); break;
+    case 2: spmv_vector_kernel<T, 2><<<blocks, 256, 0, st>>>(rows, rp, ri, rx, x, y); break;
+    case 4: spmv_vector_kernel<T, 4><<<blocks, 256, 0, st>>>(rows, rp, ri, rx, x, y); break;
+    case 8: spmv_vector_kernel<T, 8><<<blocks, 256, 0, st>>>(rows, rp, ri, rx, x, y); break;
+    case 16: spmv_vector_kernel<T, 16><<<blocks, 256, 0, st>>>(rows, rp, ri, rx, x, y); break;
+    case 32: spmv_vector_kernel<T, 32><<<blocks, 256, 0, st>>>(rows, rp, ri, rx, x, y); break;
+    case 64: spmv_vector_kernel<T, 64><<<blocks, 256, 0, st>>>(rows, rp, ri, rx, x, y); break;
+    default: return fail(1, "lanes_per_row must be a power of two in [1, 64]");
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+__global__ void primal_step_kernel(double* __restrict__ xnew, const double* __restrict__ x,
+                                   const double* __restrict__ c, const double* __restrict__ aty,
+                                   const double* __restrict__ lo, const double* __restrict__ hi,
+                                   double tau, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  xnew[i] = clampT(x[i] - tau * (c[i] + aty[i]), lo[i], hi[i]);
+}
+
+__global__ void dual_step_kernel(double* __restrict__ ynew, const double* __restrict__ y,
+                                 const double* __restrict__ ax, const double* __restrict__ rl,
+                                 const double* __restrict__ ru, double sigma, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double w = y[i] + sigma * ax[i];
+  ynew[i] = w - sigma * clampT(w / sigma, rl[i], ru[i]);
+}
+
+__global__ void pdhg_primal_kernel(double* __restrict__ x, double* __restrict__ dx,
+                                   const double* __restrict__ c, const double* __restrict__ aty,
+                                   const double* __restrict__ lo, const double* __restrict__ hi,
+                                   double tau, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = x[i];
+  const double v = clampT(xi - tau * (c[i] + aty[i]), lo[i], hi[i]);
+  dx[i] = v - xi;
+  x[i] = v;
+}
+
+__global__ void pdhg_dual_kernel(double* __restrict__ y, double* __restrict__ dy,
+                                 double* __restrict__ ax, const double* __restrict__ adx,
+                                 const double* __restrict__ rl, const double* __restrict__ ru,
+                                 double sigma, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double a = adx[i];
+  const double axn = ax[i] + a;
+  ax[i] = axn;
+  const double yo = y[i];
+  const double w = yo + sigma * (axn + a);
+  const double yv = w - sigma * clampT(w / sigma, rl[i], ru[i]);
+  dy[i] = yv - yo;
+  y[i] = yv;
+}
+
+__global__ void pdhg_accum_kernel(double* __restrict__ acc, const double* __restrict__ inc,
+                                  double* __restrict__ sum, const double* __restrict__ v,
+                                  double eta, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  acc[i] += inc[i];
+  sum[i] += eta * v[i];
+}
+
+// one 1024-thread workgroup; OP 0 = dot, 1 = 2-norm, 2 = inf-norm
+template <class T, int OP>
+__global__ void __launch_bounds__(1024) reduce_kernel(const T* __restrict__ a,
+                                                      const T* __restrict__ b, long long n,
+                                                      double* __restrict__ out) {
+  __shared__ double red[(1024 / 64 + 2) * kMaxRed];
+  double v[1] = {OP == 2 ? 0.0 : 0.0};
+  for (long long i = threadIdx.x; i < n; i += 1024) {
+    const double ai = (double)a[i];
+    if (OP == 0) v[0] += ai * (double)b[i];
+    else if (OP == 1) v[0] += ai * ai;
+    else v[0] = nanmax(v[0], fabs(ai));
+  }
+  block_reduce<1024, 1>(v, OP == 2 ? 1u : 0u, red);
+  if (threadIdx.x == 0) *out = OP == 1 ? sqrt(v[0]) : v[0];
+}
+
+template <class T, int OP>
+int launch_reduce(const T* a, const T* b, int64_t n, double* out, hipStream_t st) {
+  if (n < 0) return fail(1, "n < 0");
+  reduce_kernel<T, OP><<<1, 1024, 0, st>>>(a, b, (long long)n, out);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- tiled SpMV
+template <class T, int BLOCK, int TNNZ>
+__global__ void __launch_bounds__(BLOCK)
+    spmv_tiled_kernel(const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                      const T* __restrict__ val, const int32_t* __restrict__ tiles,
+                      const int32_t* __restrict__ part, const T* __restrict__ x,
+                      T* __restrict__ y) {
+  __shared__ __align__(16) T lds[TNNZ];
+  const int t0 = part[blockIdx.x], t1 = part[blockIdx.x + 1];
+  auto prod = [&](int q, int j, T (&v)[1]) { v[0] = val[q] * x[j]; };
+  auto epi = [&](int r, const T (&s)[1]) { y[r] = s[0]; };
+  tile_pass<BLOCK, TNNZ, 1, false, T>(ptr, idx, tiles, t0, t1, prod, epi, lds);
+}
+
+// ---------------------------------------------------------------- workspace
+struct Carver {
+  char* base;
+  size_t off;
+  size_t cap;
+  template <class U>
+  U* take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    U* p = reinterpret_cast<U*>(base ? base + off : nullptr);
+    off += sizeof(U) * (count ? count : 1);
+    return p;
+  }
+};
+
+template <class T>
+void carve_lp(Carver& cv, LpDev<T>& L, int m, int n, long long nnz, long long hist_cap) {
+  L.sav = cv.take<T>(nnz);
+  L.stv = cv.take<T>(nnz);
+  L.rowf = cv.take<double>(m);
+  L.colf = cv.take<double>(n);
+  L.tmpm = cv.take<double>(m);
+  L.tmpn = cv.take<double>(n);
+  T** cols[] = {&L.c, &L.lo, &L.hi, &L.x, &L.xn, &L.dx, &L.aty, &L.sumx, &L.xr, &L.atyf, &L.atya};
+  for (T** p : cols) *p = cv.take<T>(n);
+  T** rows[] = {&L.rl, &L.ru, &L.y, &L.yn, &L.dy, &L.ax, &L.adx, &L.sumy, &L.yr, &L.axf, &L.axa};
+  for (T** p : rows) *p = cv.take<T>(m);
+  L.xo = cv.take<double>(n);
+  L.yo = cv.take<double>(m);
+  L.dout = cv.take<double>(n);
+  L.hist = cv.take<double>(5 * (size_t)(hist_cap > 0 ? hist_cap : 1));
+  L.hist_cap = hist_cap;
+  L.st = cv.take<LpState>(1);
+}
+
+int clock_hz(double* hz) {
+  int dev = 0, khz = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  *hz = (double)khz * 1000.0;
+  return 0;
+}
+
+void host_partition(int ntiles, const int32_t* tiles, const int64_t* ptr_at_tile, int parts,
+                    int32_t* part) {
+  // ptr_at_tile[t] = pointer value at row tiles[t]; cost = nonzeros + rows
+  std::vector<double> pre(ntiles + 1, 0.0);
+  for (int t = 0; t < ntiles; ++t)
+    pre[t + 1] = pre[t] + (double)(ptr_at_tile[t + 1] - ptr_at_tile[t]) +
+                 (double)(tiles[t + 1] - tiles[t]);
+  const double total = pre[ntiles];
+  part[0] = 0;
+  int t = 0;
+  for (int p = 1; p < parts; ++p) {
+    const double target = total * (double)p / (double)parts;
+    while (t < ntiles && pre[t] < target) ++t;
+    part[p] = t;
+  }
+  part[parts] = ntiles;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sovlp_last_error(void) { return g_err.c_str(); }
+
+int sovlp_version(void) { return 100; }
+
+int sovlp_clock_rate(int device, double* hz) {
+  int khz = 0;
+  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+  *hz = (double)khz * 1000.0;
+  return 0;
+}
+
+int sovlp_tile_geometry(int precision, int* block, int* tile_nnz) {
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  *block = kBlock;
+  *tile_nnz = precision == 64 ? kTnnz64 : kTnnz32;
+  return 0;
+}
+
+int sovlp_spmv_csr_f64(int rows, const int64_t* rp, const int32_t* ri, const double* rx,
+                       const double* x, double* y, int lanes_per_row, sovlp_stream_t stream) {
+  return launch_spmv_vector<double>(rows, rp, ri, rx, x, y, lanes_per_row, S(stream));
+}
+int sovlp_spmv_csr_f32(int rows, const int64_t* rp, const int32_t* ri, const float* rx,
+                       const float* x, float* y, int lanes_per_row, sovlp_stream_t stream) {
+  return launch_spmv_vector<float>(rows, rp, ri, rx, x, y, lanes_per_row, S(stream));
+}
+
+#define ELEMWISE_LAUNCH(kernel, len, ...)                                     \
+  do {                                                                        \
+    if ((len) < 0) return fail(1, "negative length");                         \
+    if ((len) == 0) return 0;                                                 \
+    kernel<<<((len) + 255) / 256, 256, 0, S(stream)>>>(__VA_ARGS__);          \
+    HIPCHK(hipGetLastError());                                                \
+    return 0;                                                                 \
+  } while (0)
+
+int sovlp_primal_step_f64(double* xnew, const double* x, const double* c, const double* aty,
+                          const double* lo, const double* hi, double tau, int n,
+                          sovlp_stream_t stream) {
+  ELEMWISE_LAUNCH(primal_step_kernel, n, xnew, x, c, aty, lo, hi, tau, n);
+}
+int sovlp_dual_step_f64(double* ynew, const double* y, const double* ax, const double* rl,
+                        const double* ru, double sigma, int m, sovlp_stream_t stream) {
+  ELEMWISE_LAUNCH(dual_step_kernel, m, ynew, y, ax, rl, ru, sigma, m);
+}
+int sovlp_pdhg_primal_f64(double* x, double* dx, const double* c, const double* aty,
+                          const double* lo, const double* hi, double tau, int n,
+                          sovlp_stream_t stream) {
+  ELEMWISE_LAUNCH(pdhg_primal_kernel, n, x, dx, c, aty, lo, hi, tau, n);
+}
+int sovlp_pdhg_dual_f64(double* y, double* dy, double* ax, const double* adx, const double* rl,
+                        const double* ru, double sigma, int m, sovlp_stream_t stream) {
+  ELEMWISE_LAUNCH(pdhg_dual_kernel, m, y, dy, ax, adx, rl, ru, sigma, m);
+}
+int sovlp_pdhg_accum_f64(double* acc, const double* inc, double* sum, const double* v,
+                         double eta, int n, sovlp_stream_t stream) {
+  ELEMWISE_LAUNCH(pdhg_accum_kernel, n, acc, inc, sum, v, eta, n);
+}
+
+int sovlp_dot_f64(const double* a, const double* b, int64_t n, double* out, sovlp_stream_t s) {
+  return launch_reduce<double, 0>(a, b, n, out, S(s));
+}
+int sovlp_nrm2_f64(const double* a, int64_t n, double* out, sovlp_stream_t s) {
+  return launch_reduce<double, 1>(a, a, n, out, S(s));
+}
+int sovlp_nrminf_f64(const double* a, int64_t n, double* out, sovlp_stream_t s) {
+  return launch_reduce<double, 2>(a, a, n, out, S(s));
+}
+int sovlp_dot_f32(const float* a, const float* b, int64_t n, double* out, sovlp_stream_t s) {
+  return launch_reduce<float, 0>(a, b, n, out, S(s));
+}
+int sovlp_nrm2_f32(const float* a, int64_t n, double* out, sovlp_stream_t s) {
+  return launch_reduce<float, 1>(a, a, n, out, S(s));
+}
+
+int sovlp_plan_tiles(int rows, const int64_t* ptr, int tile_nnz, int tile_rows,
+                     int32_t* tiles, int32_t* ntiles_out) {
+  if (rows < 0 || tile_nnz < 1 || tile_rows < 1) return fail(1, "bad tile plan arguments");
+  if (ptr[rows] >= (int64_t)INT32_MAX) return fail(4, "more than 2^31-1 nonzeros");
+  int t = 0, r = 0;
+  tiles[0] = 0;
+  while (r < rows) {
+    const int64_t len = ptr[r + 1] - ptr[r];
+    if (len < 0) return fail(1, "row pointers must be non-decreasing");
+    if (len > tile_nnz) {
+      tiles[++t] = ++r;
+      continue;
+    }
+    int64_t acc = 0;
+    const int start = r;
+    while (r < rows && r - start < tile_rows) {
+      const int64_t l = ptr[r + 1] - ptr[r];
+      if (l > tile_nnz || acc + l > tile_nnz) break;
+      acc += l;
+      ++r;
+    }
+    tiles[++t] = r;
+  }
+  *ntiles_out = t;
+  return 0;
+}
+
+int sovlp_partition_tiles(int ntiles, const int32_t* tiles, const int64_t* ptr, int parts,
+                          int32_t* part_out) {
+  if (ntiles < 0 || parts < 1) return fail(1, "bad partition arguments");
+  std::vector<int64_t> pt(ntiles + 1);
+  for (int t = 0; t <= ntiles; ++t) pt[t] = ptr[tiles[t]];
+  host_partition(ntiles, tiles, pt.data(), parts, part_out);
+  return 0;
+}
+
+}  // extern "C"
+
+template <class T, int TNNZ>
+static int spmv_tiled(int rows, const int32_t* ptr32, const int32_t* idx, const T* val,
+                      const int32_t* tiles, int ntiles, const int32_t* part, int grid,
+                      const T* x, T* y, hipStream_t st) {
+  if (rows < 0 || ntiles < 0 || grid < 1) return fail(1, "bad tiled SpMV arguments");
+  if (rows == 0 || ntiles == 0) return 0;
+  spmv_tiled_kernel<T, kBlock, TNNZ><<<grid, kBlock, 0, st>>>(ptr32, idx, val, tiles, part, x, y);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" {
+
+int sovlp_spmv_tiled_f64(int rows, const int32_t* ptr32, const int32_t* idx, const double* val,
+                         const int32_t* tiles, int ntiles, const int32_t* part, int grid,
+                         const double* x, double* y, sovlp_stream_t stream) {
+  return spmv_tiled<double, kTnnz64>(rows, ptr32, idx, val, tiles, ntiles, part, grid, x, y,
+                                     S(stream));
+}
+int sovlp_spmv_tiled_f32(int rows, const int32_t* ptr32, const int32_t* idx, const float* val,
+                         const int32_t* tiles, int ntiles, const int32_t* part, int grid,
+                         const float* x, float* y, sovlp_stream_t stream) {
+  return spmv_tiled<float, kTnnz32>(rows, ptr32, idx, val, tiles, ntiles, part, grid, x, y,
+                                    S(stream));
+}
+
+int sovlp_selftest(void) {
+  // [[1, 0, 2], [0, 3, 0]] . [1, 2, 3] = [7, 6]
+  const int64_t rp[3] = {0, 2, 3};
+  const int32_t rp32[3] = {0, 2, 3}, ri[3] = {0, 2, 1}, tiles[2] = {0, 2}, part[2] = {0, 1};
+  const double rx[3] = {1.0, 2.0, 3.0}, xv[3] = {1.0, 2.0, 3.0};
+  const float rxf[3] = {1.0f, 2.0f, 3.0f}, xf[3] = {1.0f, 2.0f, 3.0f};
+  char* d = nullptr;
+  HIPCHK(hipMalloc(&d, 4096));
+  int64_t* d_rp = (int64_t*)d;
+  int32_t* d_rp32 = (int32_t*)(d + 64);
+  int32_t* d_ri = (int32_t*)(d + 128);
+  int32_t* d_tiles = (int32_t*)(d + 192);
+  int32_t* d_part = (int32_t*)(d + 256);
+  double* d_rx = (double*)(d + 320);
+  double* d_x = (double*)(d + 384);
+  double* d_y = (double*)(d + 448);
+  float* d_rxf = (float*)(d + 512);
+  float* d_xf = (float*)(d + 576);
+  float* d_yf = (float*)(d + 640);
+  int rc = 0;
+  std::string why;
+  auto run = [&]() -> int {
+    HIPCHK(hipMemcpy(d_rp, rp, sizeof rp, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_rp32, rp32, sizeof rp32, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ri, ri, sizeof ri, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_tiles, tiles, sizeof tiles, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_part, part, sizeof part, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_rx, rx, sizeof rx, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_x, xv, sizeof xv, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_rxf, rxf, sizeof rxf, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_xf, xf, sizeof xf, hipMemcpyHostToDevice));
+    for (int variant = 0; variant < 4; ++variant) {
+      double got[2] = {12345.0, 12345.0};
+      float gotf[2] = {12345.0f, 12345.0f};
+      HIPCHK(hipMemcpy(d_y, got, sizeof got, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_yf, gotf, sizeof gotf, hipMemcpyHostToDevice));
+      int r = 0;
+      if (variant == 0) r = sovlp_spmv_csr_f64(2, d_rp, d_ri, d_rx, d_x, d_y, 64, nullptr);
+      if (variant == 1) r = sovlp_spmv_tiled_f64(2, d_rp32, d_ri, d_rx, d_tiles, 1, d_part, 1, d_x, d_y, nullptr);
+      if (variant == 2) r = sovlp_spmv_csr_f32(2, d_rp, d_ri, d_rxf, d_xf, d_yf, 4, nullptr);
+      if (variant == 3) r = sovlp_spmv_tiled_f32(2, d_rp32, d_ri, d_rxf, d_tiles, 1, d_part, 1, d_xf, d_yf, nullptr);
+      if (r) return r;
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipMemcpy(got, d_y, sizeof got, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(gotf, d_yf, sizeof gotf, hipMemcpyDeviceToHost));
+      const bool ok = variant < 2 ? (got[0] == 7.0 && got[1] == 6.0)
+                                  : (gotf[0] == 7.0f && gotf[1] == 6.0f);
+      if (!ok) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "selftest variant %d returned [%g, %g], expected [7, 6]",
+                      variant, variant < 2 ? got[0] : gotf[0], variant < 2 ? got[1] : gotf[1]);
+        why = buf;
+        return 5;
+      }
+    }
+    return 0;
+  };
+  rc = run();
+  (void)hipFree(d);
+  if (rc == 5) return fail(5, why);
+  return rc;
+}
+
+}  // extern "C"
+
+// ====================================================================== engines
+struct sovlp_pdlp {
+  int precision, W, m, n;
+  void* lpdev;               // LpDev<T> (device)
+  LpState* st;
+  StepCtl* ctl;
+  unsigned* done;
+  double *xo, *yo, *dout, *hist;
+  long long hist_cap;
+  double hz;
+  bool started;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  hipStream_t cap;
+  long long ticks_set;
+};
+
+struct sovlp_batch {
+  int count, precision, block;
+  void* lps;                 // LpDev<T>[count] (device)
+  StepCtl* ctl;
+  unsigned* done;
+  std::vector<double*> xo, yo, dout;
+  std::vector<LpState*> st;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  hipStream_t cap;
+  double hz;
+  long long ticks_set;
+};
+
+namespace {
+
+constexpr int kTnnzSmall = 512;   // 64-thread workgroups: 2 x 512 x 8 B = 8 KiB LDS
+constexpr int kStepsPerGraph = 64;
+
+template <class T>
+void fill_lp(LpDev<T>& L, const sovlp_lp& lp) {
+  L.m = lp.m; L.n = lp.n; L.ntA = lp.ntiles_a; L.ntT = lp.ntiles_t;
+  L.flip = lp.maximise ? 1 : 0;
+  L.offset0 = lp.obj_offset;
+  L.ap = lp.rp; L.ai = lp.ri; L.av = lp.rx;
+  L.tp = lp.cp; L.ti = lp.ci; L.tv = lp.cx;
+  L.c0 = lp.c; L.lo0 = lp.lo; L.hi0 = lp.hi; L.rl0 = lp.rl; L.ru0 = lp.ru; L.v0 = lp.v0;
+  L.tilesA = lp.tiles_a;
+  L.tilesT = lp.tiles_t;
+}
+
+// workgroups sharing one LP's phases: one per ~4 tiles, at most what is resident
+template <class T>
+int pick_width(const sovlp_lp* lp, int team, int* W) {
+  int dev = 0, cus = 0, nb = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, pdlp_phase_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>, kBlock, 0));
+  const long long tiles = (long long)lp->ntiles_a + lp->ntiles_t;
+  const long long resident = (long long)std::max(nb, 1) * cus;
+  long long w = std::max<long long>(1, tiles / 4);
+  if (team == 1) w = 1;
+  if (team == 0 && lp->nnz < 200000) w = 1;
+  *W = (int)std::min(w, resident);
+  return 0;
+}
+
+template <class T>
+size_t single_ws_T(const sovlp_lp* lp, long long hist_cap, int W) {
+  Carver cv{nullptr, 0, 0};
+  cv.take<LpDev<T>>(1);
+  cv.take<StepCtl>(1);
+  cv.take<unsigned>(64);
+  cv.take<double>((size_t)W * kMaxRed);
+  cv.take<int32_t>(W + 1);
+  cv.take<int32_t>(W + 1);
+  LpDev<T> L{};
+  carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
+  return cv.off + 256;
+}
+
+template <class T>
+int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64_t ws_bytes,
+                    int64_t hist_cap, int team, hipStream_t st, sovlp_pdlp* h) {
+  int W = 1;
+  int rc = pick_width<T>(lp, team, &W);
+  if (rc) return rc;
+  if ((size_t)ws_bytes < single_ws_T<T>(lp, hist_cap, W)) return fail(1, "workspace too small");
+  Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
+  LpDev<T>* d_lp = cv.take<LpDev<T>>(1);
+  StepCtl* ctl = cv.take<StepCtl>(1);
+  unsigned* done = cv.take<unsigned>(64);
+  double* partials = cv.take<double>((size_t)W * kMaxRed);
+  int32_t* partA = cv.take<int32_t>(W + 1);
+  int32_t* partT = cv.take<int32_t>(W + 1);
+  LpDev<T> L{};
+  carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
+  fill_lp<T>(L, *lp);
+  L.partials = partials;
+  if (W > 1) {
+    // equal-cost split of both tile lists (needs the pointer values at tile starts)
+    std::vector<int32_t> ta(lp->ntiles_a + 1), tt(lp->ntiles_t + 1), rp(lp->m + 1), cp(lp->n + 1);
+    HIPCHK(hipMemcpyAsync(ta.data(), lp->tiles_a, sizeof(int32_t) * ta.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(tt.data(), lp->tiles_t, sizeof(int32_t) * tt.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(rp.data(), lp->rp, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cp.data(), lp->cp, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int64_t> pa(ta.size()), pt(tt.size());
+    for (size_t t = 0; t < ta.size(); ++t) pa[t] = rp[ta[t]];
+    for (size_t t = 0; t < tt.size(); ++t) pt[t] = cp[tt[t]];
+    std::vector<int32_t> ha(W + 1), ht(W + 1);
+    host_partition(lp->ntiles_a, ta.data(), pa.data(), W, ha.data());
+    host_partition(lp->ntiles_t, tt.data(), pt.data(), W, ht.data());
+    HIPCHK(hipMemcpyAsync(partA, ha.data(), sizeof(int32_t) * (W + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(partT, ht.data(), sizeof(int32_t) * (W + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    L.partA = partA;
+    L.partT = partT;
+  }
+  LpState s0{};
+  s0.phase = PH_INIT;
+  s0.status = ST_RUNNING;
+  StepCtl c0{};
+  std::memcpy(&c0.P, prm, sizeof(Params));
+  c0.ticks = (long long)9.0e18;
+  HIPCHK(hipMemcpyAsync(L.st, &s0, sizeof s0, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(done, 0, sizeof(unsigned), st));
+  HIPCHK(hipMemcpyAsync(d_lp, &L, sizeof L, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+
+  h->precision = sizeof(T) == 8 ? 64 : 32;
+  h->W = W;
+  h->m = lp->m;
+  h->n = lp->n;
+  h->lpdev = d_lp;
+  h->st = L.st;
+  h->ctl = ctl;
+  h->done = done;
+  h->xo = L.xo; h->yo = L.yo; h->dout = L.dout; h->hist = L.hist;
+  h->hist_cap = hist_cap;
+  h->ticks_set = c0.ticks;
+  // record kStepsPerGraph phases into a graph on a private capture stream
+  HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+  HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+  constexpr int TN = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
+  for (int i = 0; i < kStepsPerGraph; ++i) {
+    if (W == 1) {
+      pdlp_step_kernel<T, kBlock, TN><<<1, kBlock, 0, h->cap>>>(d_lp, ctl, done);
+    } else {
+      pdlp_phase_kernel<T, kBlock, TN><<<W, kBlock, 0, h->cap>>>(d_lp);
+      pdlp_decide_kernel<T, kBlock><<<1, kBlock, 0, h->cap>>>(d_lp, W, ctl, done);
+    }
+  }
+  HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
+  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  return clock_hz(&h->hz);
+}
+
+template <class T>
+size_t batch_ws_T(int count, const sovlp_lp* lps, long long hist_cap) {
+  Carver cv{nullptr, 0, 0};
+  cv.take<LpDev<T>>(count);
+  cv.take<StepCtl>(1);
+  cv.take<unsigned>(64);
+  for (int i = 0; i < count; ++i) {
+    LpDev<T> L{};
+    carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
+  }
+  return cv.off + 256;
+}
+
+template <class T>
+int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, int block, void* ws,
+                   int64_t ws_bytes, int64_t hist_cap, hipStream_t st, sovlp_batch* h) {
+  if ((size_t)ws_bytes < batch_ws_T<T>(count, lps, hist_cap)) return fail(1, "workspace too small");
+  Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
+  LpDev<T>* d_lps = cv.take<LpDev<T>>(count);
+  StepCtl* ctl = cv.take<StepCtl>(1);
+  unsigned* done = cv.take<unsigned>(64);
+  std::vector<LpDev<T>> host(count);
+  h->xo.resize(count);
+  h->yo.resize(count);
+  h->dout.resize(count);
+  h->st.resize(count);
+  for (int i = 0; i < count; ++i) {
+    if (lps[i].nnz >= (int64_t)INT32_MAX) return fail(4, "an LP has more than 2^31-1 nonzeros");
+    LpDev<T>& L = host[i];
+    L = LpDev<T>{};
+    carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
+    fill_lp<T>(L, lps[i]);
+    h->xo[i] = L.xo; h->yo[i] = L.yo; h->dout[i] = L.dout; h->st[i] = L.st;
+  }
+  StepCtl c0{};
+  std::memcpy(&c0.P, prm, sizeof(Params));
+  c0.ticks = (long long)9.0e18;
+  HIPCHK(hipMemcpyAsync(d_lps, host.data(), sizeof(LpDev<T>) * count, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  h->count = count;
+  h->precision = sizeof(T) == 8 ? 64 : 32;
+  h->block = block;
+  h->lps = d_lps;
+  h->ctl = ctl;
+  h->done = done;
+  h->ticks_set = c0.ticks;
+  HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+  HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < kStepsPerGraph; ++i) {
+    if (block == 64)
+      pdlp_step_kernel<T, 64, kTnnzSmall><<<count, 64, 0, h->cap>>>(d_lps, ctl, done);
+    else
+      pdlp_step_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>
+          <<<count, kBlock, 0, h->cap>>>(d_lps, ctl, done);
+  }
+  HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
+  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  return clock_hz(&h->hz);
+}
+
+int set_ticks(StepCtl* ctl, long long* cached, double seconds, double hz, hipStream_t st) {
+  const double secs = seconds > 0.0 ? seconds : 0.0;
+  long long ticks = (long long)std::min(secs * hz, 9.0e18);
+  if (ticks != *cached) {
+    *cached = ticks;
+    HIPCHK(hipMemcpyAsync(&ctl->ticks, cached, sizeof(long long), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));   // the source is host memory owned by the handle
+  }
+  return 0;
+}
+
+void fill_info(const LpState& t, sovlp_info* f) {
+  std::memset(f, 0, sizeof *f);
+  const bool opt = t.status == ST_OPTIMAL;
+  const double nan = std::nan("");
+  f->status = t.status;
+  f->stage = t.phase == PH_DONE ? 3 : (t.phase < PH_KKT0 ? 0 : (t.phase >= PH_FINAL ? 2 : 1));
+  f->iterations = t.k;
+  f->restarts = t.restarts;
+  f->rejected_steps = t.rejected;
+  f->history_len = t.hist_len;
+  f->primal_res = opt ? t.fin[0] : nan;
+  f->dual_res = opt ? t.fin[1] : nan;
+  f->gap = opt ? t.fin[2] : nan;
+  f->primal_obj = opt ? t.fin[3] : nan;
+  f->dual_obj = opt ? t.fin[4] : nan;
+  f->step_size = t.eta;
+  f->primal_weight = t.omega;
+  f->objective = t.objective;
+  f->scale_obj = t.sobj;
+  f->device_ticks = t.ticks;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sovlp_pdlp_workspace_bytes(const sovlp_lp* lp, int precision, int64_t history_cap,
+                               int64_t* bytes) {
+  if (!lp || !bytes) return fail(1, "null argument");
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  int W = 1;
+  int rc = precision == 64 ? pick_width<double>(lp, 2, &W) : pick_width<float>(lp, 2, &W);
+  if (rc) return rc;
+  *bytes = (int64_t)(precision == 64 ? single_ws_T<double>(lp, history_cap, W)
+                                     : single_ws_T<float>(lp, history_cap, W));
+  return 0;
+}
+
+int sovlp_pdlp_create(const sovlp_lp* lp, const sovlp_params* params, int precision,
+                      void* workspace, int64_t workspace_bytes, int64_t history_cap, int team,
+                      sovlp_stream_t stream, sovlp_pdlp** out) {
+  static_assert(sizeof(sovlp_params) == sizeof(Params), "params layout");
+  if (!lp || !params || !workspace || !out) return fail(1, "null argument");
+  if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) return fail(1, "negative dimension");
+  if (lp->nnz >= (int64_t)INT32_MAX) return fail(4, "more than 2^31-1 nonzeros");
+  if (params->check_every < 1 || params->stepsize_every < 1) return fail(1, "check/stepsize period < 1");
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  if (team < 0 || team > 2) return fail(1, "team must be 0 (auto), 1 (one workgroup) or 2 (many)");
+  sovlp_pdlp* h = new sovlp_pdlp{};
+  int rc = precision == 64
+               ? single_create_T<double>(lp, params, workspace, workspace_bytes, history_cap, team,
+                                         S(stream), h)
+               : single_create_T<float>(lp, params, workspace, workspace_bytes, history_cap, team,
+                                        S(stream), h);
+  if (rc) {
+    sovlp_pdlp_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int sovlp_pdlp_advance(sovlp_pdlp* h, int64_t iteration_budget, double seconds_left,
+                       sovlp_stream_t stream) {
+  if (!h) return fail(1, "null handle");
+  hipStream_t st = S(stream);
+  if (!h->started) {
+    int rc = set_ticks(h->ctl, &h->ticks_set, seconds_left, h->hz, st);
+    if (rc) return rc;
+    h->started = true;
+  }
+  LpState s;
+  HIPCHK(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const long long k0 = s.k;
+  const long long budget = iteration_budget < 1 ? 1 : iteration_budget;
+  // every replay runs kStepsPerGraph phases; an iteration takes at most 61
+  // phases (60 rejected trials), so this bound is never the one that stops a
+  // healthy solve
+  const long long max_replays = (budget * 61 + 200) / kStepsPerGraph + 16;
+  long long replays = 0;
+  while (s.phase != PH_DONE && s.k - k0 < budget && replays++ < max_replays) {
+    HIPCHK(hipGraphLaunch(h->exec, st));
+    HIPCHK(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+int sovlp_pdlp_info(sovlp_pdlp* h, sovlp_info* info, sovlp_stream_t stream) {
+  if (!h || !info) return fail(1, "null argument");
+  LpState s;
+  HIPCHK(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, S(stream)));
+  HIPCHK(hipStreamSynchronize(S(stream)));
+  fill_info(s, info);
+  return 0;
+}
+
+int sovlp_pdlp_outputs(sovlp_pdlp* h, double** x, double** y, double** d, double** history) {
+  if (!h) return fail(1, "null handle");
+  if (x) *x = h->xo;
+  if (y) *y = h->yo;
+  if (d) *d = h->dout;
+  if (history) *history = h->hist;
+  return 0;
+}
+
+int sovlp_pdlp_width(sovlp_pdlp* h, int* workgroups) {
+  if (!h || !workgroups) return fail(1, "null argument");
+  *workgroups = h->W;
+  return 0;
+}
+
+int sovlp_pdlp_destroy(sovlp_pdlp* h) {
+  if (!h) return 0;
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
+  delete h;
+  return 0;
+}
+
+int sovlp_batch_geometry(int precision, int block, int* tile_rows, int* tile_nnz) {
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  if (block != 64 && block != 256) return fail(1, "block must be 64 or 256");
+  *tile_rows = block;
+  *tile_nnz = block == 64 ? kTnnzSmall : (precision == 64 ? kTnnz64 : kTnnz32);
+  return 0;
+}
+
+int sovlp_batch_workspace_bytes(int count, const sovlp_lp* lps, int precision,
+                                int64_t history_cap, int64_t* bytes) {
+  if (count < 1 || !lps || !bytes) return fail(1, "bad batch arguments");
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  *bytes = (int64_t)(precision == 64 ? batch_ws_T<double>(count, lps, history_cap)
+                                     : batch_ws_T<float>(count, lps, history_cap));
+  return 0;
+}
+
+int sovlp_batch_create(int count, const sovlp_lp* lps, const sovlp_params* params,
+                       int precision, int block, void* workspace, int64_t workspace_bytes,
+                       int64_t history_cap, sovlp_stream_t stream, sovlp_batch** out) {
+  if (count < 1 || !lps || !params || !workspace || !out) return fail(1, "bad batch arguments");
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  if (block != 64 && block != 256) return fail(1, "block must be 64 or 256");
+  if (params->check_every < 1 || params->stepsize_every < 1) return fail(1, "check/stepsize period < 1");
+  sovlp_batch* h = new sovlp_batch{};
+  int rc = precision == 64
+               ? batch_create_T<double>(count, lps, params, block, workspace, workspace_bytes,
+                                        history_cap, S(stream), h)
+               : batch_create_T<float>(count, lps, params, block, workspace, workspace_bytes,
+                                       history_cap, S(stream), h);
+  if (rc) {
+    sovlp_batch_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int sovlp_batch_run(sovlp_batch* h, double seconds_per_lp, int64_t max_graphs,
+                    sovlp_stream_t stream, int64_t* graphs_run) {
+  if (!h) return fail(1, "null handle");
+  hipStream_t st = S(stream);
+  int rc = set_ticks(h->ctl, &h->ticks_set, seconds_per_lp, h->hz, st);
+  if (rc) return rc;
+  if (h->precision == 64)
+    state_reset_kernel<double><<<(h->count + 255) / 256, 256, 0, st>>>(
+        (const LpDev<double>*)h->lps, h->count, h->done);
+  else
+    state_reset_kernel<float><<<(h->count + 255) / 256, 256, 0, st>>>(
+        (const LpDev<float>*)h->lps, h->count, h->done);
+  HIPCHK(hipGetLastError());
+  unsigned done = 0;
+  long long g = 0;
+  while (done < (unsigned)h->count && (max_graphs <= 0 || g < max_graphs)) {
+    HIPCHK(hipGraphLaunch(h->exec, st));
+    ++g;
+    HIPCHK(hipMemcpyAsync(&done, h->done, sizeof done, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (graphs_run) *graphs_run = g;
+  return 0;
+}
+
+int sovlp_batch_infos(sovlp_batch* h, sovlp_info* infos, sovlp_stream_t stream) {
+  if (!h || !infos) return fail(1, "null argument");
+  std::vector<LpState> s(h->count);
+  for (int i = 0; i < h->count; ++i)
+    HIPCHK(hipMemcpyAsync(&s[i], h->st[i], sizeof(LpState), hipMemcpyDeviceToHost, S(stream)));
+  HIPCHK(hipStreamSynchronize(S(stream)));
+  for (int i = 0; i < h->count; ++i) fill_info(s[i], &infos[i]);
+  return 0;
+}
+
+int sovlp_batch_outputs(sovlp_batch* h, int index, double** x, double** y, double** d) {
+  if (!h || index < 0 || index >= h->count) return fail(1, "bad batch index");
+  if (x) *x = h->xo[index];
+  if (y) *y = h->yo[index];
+  if (d) *d = h->dout[index];
+  return 0;
+}
+
+int sovlp_batch_destroy(sovlp_batch* h) {
+  if (!h) return 0;
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
+  delete h;
+  return 0;
+}
+
+}  // extern "C"

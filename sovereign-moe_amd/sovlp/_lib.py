@@ -1,0 +1,121 @@
+"""ctypes binding of ``libsovlp.so`` (declarations in ``include/sovlp.h``).
+
+The library is built in-tree (``__graft_entry__.build()`` or ``make -C
+sovereign-moe_amd/csrc``). Loading fails loudly when it is missing: there is no
+CPU fallback anywhere in the package.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsovlp.so")
+
+_lib = None
+
+P = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+F64 = C.c_double
+
+
+class SovlpError(RuntimeError):
+    """A nonzero return code of the C ABI, with the library's message."""
+
+
+class sovlp_params(C.Structure):
+    _fields_ = [("eps_abs", F64), ("eps_rel", F64), ("feas_cap", F64), ("max_iter", I64),
+                ("time_limit", F64), ("check_every", I32), ("stepsize_every", I32),
+                ("beta_sufficient", F64), ("beta_necessary", F64), ("beta_artificial", F64),
+                ("primal_weight_smoothing", F64)]
+
+
+class sovlp_lp(C.Structure):
+    _fields_ = [("m", I32), ("n", I32), ("nnz", I64),
+                ("rp", P), ("ri", P), ("rx", P), ("cp", P), ("ci", P), ("cx", P),
+                ("c", P), ("rl", P), ("ru", P), ("lo", P), ("hi", P), ("v0", P),
+                ("tiles_a", P), ("ntiles_a", I32), ("tiles_t", P), ("ntiles_t", I32),
+                ("obj_offset", F64), ("maximise", I32)]
+
+
+class sovlp_info(C.Structure):
+    _fields_ = [("status", I32), ("stage", I32), ("iterations", I64), ("restarts", I64),
+                ("rejected_steps", I64), ("history_len", I64), ("primal_res", F64),
+                ("dual_res", F64), ("gap", F64), ("primal_obj", F64), ("dual_obj", F64),
+                ("step_size", F64), ("primal_weight", F64), ("objective", F64),
+                ("scale_obj", F64), ("device_ticks", I64)]
+
+
+# name -> (restype, argtypes); every entry point of include/sovlp.h
+_SIGS = {
+    "sovlp_last_error": (C.c_char_p, []),
+    "sovlp_version": (C.c_int, []),
+    "sovlp_selftest": (C.c_int, []),
+    "sovlp_clock_rate": (C.c_int, [C.c_int, C.POINTER(F64)]),
+    "sovlp_spmv_csr_f64": (C.c_int, [C.c_int, P, P, P, P, P, C.c_int, P]),
+    "sovlp_spmv_csr_f32": (C.c_int, [C.c_int, P, P, P, P, P, C.c_int, P]),
+    "sovlp_primal_step_f64": (C.c_int, [P, P, P, P, P, P, F64, C.c_int, P]),
+    "sovlp_dual_step_f64": (C.c_int, [P, P, P, P, P, F64, C.c_int, P]),
+    "sovlp_pdhg_primal_f64": (C.c_int, [P, P, P, P, P, P, F64, C.c_int, P]),
+    "sovlp_pdhg_dual_f64": (C.c_int, [P, P, P, P, P, P, F64, C.c_int, P]),
+    "sovlp_pdhg_accum_f64": (C.c_int, [P, P, P, P, F64, C.c_int, P]),
+    "sovlp_dot_f64": (C.c_int, [P, P, I64, P, P]),
+    "sovlp_nrm2_f64": (C.c_int, [P, I64, P, P]),
+    "sovlp_nrminf_f64": (C.c_int, [P, I64, P, P]),
+    "sovlp_dot_f32": (C.c_int, [P, P, I64, P, P]),
+    "sovlp_nrm2_f32": (C.c_int, [P, I64, P, P]),
+    "sovlp_plan_tiles": (C.c_int, [C.c_int, P, C.c_int, C.c_int, P, C.POINTER(I32)]),
+    "sovlp_partition_tiles": (C.c_int, [C.c_int, P, P, C.c_int, P]),
+    "sovlp_tile_geometry": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sovlp_spmv_tiled_f64": (C.c_int, [C.c_int, P, P, P, P, C.c_int, P, C.c_int, P, P, P]),
+    "sovlp_spmv_tiled_f32": (C.c_int, [C.c_int, P, P, P, P, C.c_int, P, C.c_int, P, P, P]),
+    "sovlp_pdlp_workspace_bytes": (C.c_int, [C.POINTER(sovlp_lp), C.c_int, I64, C.POINTER(I64)]),
+    "sovlp_pdlp_create": (C.c_int, [C.POINTER(sovlp_lp), C.POINTER(sovlp_params), C.c_int, P,
+                                    I64, I64, C.c_int, P, C.POINTER(P)]),
+    "sovlp_pdlp_advance": (C.c_int, [P, I64, F64, P]),
+    "sovlp_pdlp_info": (C.c_int, [P, C.POINTER(sovlp_info), P]),
+    "sovlp_pdlp_outputs": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+    "sovlp_pdlp_width": (C.c_int, [P, C.POINTER(C.c_int)]),
+    "sovlp_pdlp_destroy": (C.c_int, [P]),
+    "sovlp_batch_geometry": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sovlp_batch_workspace_bytes": (C.c_int, [C.c_int, P, C.c_int, I64, C.POINTER(I64)]),
+    "sovlp_batch_create": (C.c_int, [C.c_int, P, C.POINTER(sovlp_params), C.c_int, C.c_int, P,
+                                     I64, I64, P, C.POINTER(P)]),
+    "sovlp_batch_run": (C.c_int, [P, F64, I64, P, C.POINTER(I64)]),
+    "sovlp_batch_infos": (C.c_int, [P, P, P]),
+    "sovlp_batch_outputs": (C.c_int, [P, C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+    "sovlp_batch_destroy": (C.c_int, [P]),
+}
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def lib():
+    """The loaded library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libsovlp.so is not built ({LIB_PATH}); run "
+                               "`python -c 'import __graft_entry__ as g; g.build()'` "
+                               "or `make -C sovereign-moe_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().sovlp_last_error().decode(errors="replace")
+        raise SovlpError(f"{what or 'sovlp'} failed (code {rc}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)

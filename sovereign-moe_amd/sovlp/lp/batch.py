@@ -1,0 +1,191 @@
+"""Batched PDLP: many independent, uneven LPs solved together on one GPU.
+
+No reference counterpart: SOVOPT solves one LP per ``solve_pdlp`` call
+(``src/sovopt/lp/pdlp.py:238``); its only batched PDHG (``src/sovopt/mip/bnr.py:176-301``)
+shares one ``A`` across branch-and-bound nodes. Here every LP keeps its own
+matrix, scaling, step size, restarts and termination test — exactly the
+single-LP algorithm — and the device runs one workgroup per LP per launch,
+every launch advancing each unfinished LP by one phase. The launches are
+replayed from a hipGraph until every LP has terminated.
+
+An LP's arithmetic depends only on its own data and on the workgroup size,
+which the autotuner (``sovlp.lp.autotune``) picks from the LP's own shape, so
+a result is the same whichever other LPs share the batch.
+
+All arrays of the batch live in a handful of concatenated device tensors;
+``sovlp_lp`` descriptors point into them.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from .. import _lib
+from ..core.backend import get_backend
+from ..core.problem import ObjSense, Problem, Solution, Status
+from .pdlp import PDLPInfo, PDLPParams, plan_tiles
+
+__all__ = ["BatchResult", "solve_batch", "DeviceBatch", "batch_geometry"]
+
+
+def batch_geometry(precision: int, block: int) -> tuple[int, int]:
+    r, t = C.c_int(), C.c_int()
+    _lib.call("sovlp_batch_geometry", int(precision), int(block), C.byref(r), C.byref(t))
+    return r.value, t.value
+
+
+class DeviceBatch:
+    """Uploads many LPs into concatenated device tensors and builds their
+    descriptors (minimisation form; costs negated for maximisations)."""
+
+    def __init__(self, problems: list[Problem], precision: int = 32, block: int = 256):
+        import torch
+        self.count = len(problems)
+        rows, tnnz = batch_geometry(precision, block)
+        nmax = max(p.n for p in problems)
+        v0 = np.random.default_rng(0).standard_normal(nmax)   # each LP uses a prefix
+        cat = {k: [] for k in ("rp", "ri", "rx", "cp", "ci", "cx", "c", "rl", "ru", "lo", "hi",
+                               "ta", "tt")}
+        meta = []
+        for p in problems:
+            A = p.A
+            flip = p.sense == ObjSense.MAXIMISE
+            ta = plan_tiles(A.rp, tnnz, rows)
+            tt = plan_tiles(A.cp, tnnz, rows)
+            cat["rp"].append(A.rp.astype(np.int32))
+            cat["ri"].append(A.ri)
+            cat["rx"].append(A.rx)
+            cat["cp"].append(A.cp.astype(np.int32))
+            cat["ci"].append(A.ci)
+            cat["cx"].append(A.cx)
+            cat["c"].append(-p.c if flip else p.c)
+            cat["rl"].append(p.row_lb)
+            cat["ru"].append(p.row_ub)
+            cat["lo"].append(p.col_lb)
+            cat["hi"].append(p.col_ub)
+            cat["ta"].append(ta)
+            cat["tt"].append(tt)
+            meta.append((A.m, A.n, A.nnz, ta.size - 1, tt.size - 1,
+                         -p.obj_offset if flip else p.obj_offset, int(flip)))
+        dt = {"rp": np.int32, "ri": np.int32, "cp": np.int32, "ci": np.int32, "ta": np.int32,
+              "tt": np.int32}
+        self.t = {}
+        offs = {}
+        for k, parts in cat.items():
+            arr = np.concatenate(parts).astype(dt.get(k, np.float64), copy=False)
+            self.t[k] = torch.from_numpy(np.ascontiguousarray(arr)).to("cuda")
+            offs[k] = np.concatenate([[0], np.cumsum([a.size for a in parts])])
+        self.t["v0"] = torch.from_numpy(v0).to("cuda")
+        descs = (_lib.sovlp_lp * self.count)()
+        for i, (m, n, nnz, nta, ntt, off, flip) in enumerate(meta):
+            def ptr(k, elem):
+                return self.t[k].data_ptr() + int(offs[k][i]) * elem
+            descs[i] = _lib.sovlp_lp(m, n, nnz, ptr("rp", 4), ptr("ri", 4), ptr("rx", 8),
+                                     ptr("cp", 4), ptr("ci", 4), ptr("cx", 8), ptr("c", 8),
+                                     ptr("rl", 8), ptr("ru", 8), ptr("lo", 8), ptr("hi", 8),
+                                     self.t["v0"].data_ptr(), ptr("ta", 4), nta, ptr("tt", 4),
+                                     ntt, float(off), flip)
+        self.descs = descs
+        self.meta = meta
+
+
+@dataclass
+class BatchResult:
+    solutions: list
+    seconds: float            # wall time of the device solve (graph replays), inputs resident
+    graphs: int               # graph replays
+    lps_per_second: float
+    latency_p50: float        # per-LP device time from its first phase to its last
+    latency_p99: float
+    precision: int
+    block: int
+
+
+def solve_batch(problems: list[Problem], params: PDLPParams | None = None, precision: int = 32,
+                block: int = 256, history_cap: int = 1, repeat: int = 1,
+                keep_outputs: bool = True) -> BatchResult:
+    """Solve every LP of ``problems`` on the GPU in one batch.
+
+    ``repeat`` > 1 solves the same batch again from scratch (timing). Returns
+    per-LP ``Solution`` objects (same fields as ``solve_pdlp``'s, method
+    ``"pdlp[gpu,batch]"``) and the batch throughput.
+    """
+    import torch
+    params = params or PDLPParams()
+    get_backend(params.device)
+    db = DeviceBatch(problems, precision, block)
+    nbytes = C.c_int64()
+    _lib.call("sovlp_batch_workspace_bytes", db.count, db.descs, int(precision), int(history_cap),
+              C.byref(nbytes))
+    ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device="cuda")
+    cp = params.to_c()
+    h = C.c_void_p()
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.call("sovlp_batch_create", db.count, db.descs, C.byref(cp), int(precision), int(block),
+              ws.data_ptr(), nbytes.value, int(history_cap), stream, C.byref(h))
+    try:
+        max_graphs = (int(params.max_iter) * 61 + 400) // 64 + 16
+        g = C.c_int64()
+        best = None
+        for _ in range(max(1, repeat)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _lib.call("sovlp_batch_run", h, float(params.time_limit), max_graphs, stream,
+                      C.byref(g))
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        infos = (_lib.sovlp_info * db.count)()
+        _lib.call("sovlp_batch_infos", h, infos, stream)
+        hz = C.c_double()
+        _lib.call("sovlp_clock_rate", 0, C.byref(hz))
+        sols = []
+        if keep_outputs:
+            xs, ys, ds = [], [], []
+            base = ws.data_ptr()
+            for i in range(db.count):
+                x, y, d = C.c_void_p(), C.c_void_p(), C.c_void_p()
+                _lib.call("sovlp_batch_outputs", h, i, C.byref(x), C.byref(y), C.byref(d))
+                m, n = db.meta[i][0], db.meta[i][1]
+                xs.append(ws[x.value - base: x.value - base + 8 * n])
+                ys.append(ws[y.value - base: y.value - base + 8 * m])
+                ds.append(ws[d.value - base: d.value - base + 8 * n])
+            xall = torch.cat(xs).view(torch.float64).cpu().numpy()
+            yall = torch.cat(ys).view(torch.float64).cpu().numpy()
+            dall = torch.cat(ds).view(torch.float64).cpu().numpy()
+            ox = oy = 0
+        lat = []
+        for i in range(db.count):
+            inf = infos[i]
+            m, n = db.meta[i][0], db.meta[i][1]
+            secs = inf.device_ticks / hz.value if hz.value else float("nan")
+            lat.append(secs)
+            st = Status(int(inf.status)) if inf.status >= 0 else Status.NOT_SOLVED
+            info = PDLPInfo(iterations=int(inf.iterations), restarts=int(inf.restarts),
+                            rejected_steps=int(inf.rejected_steps), primal_res=inf.primal_res,
+                            dual_res=inf.dual_res, gap=inf.gap, primal_obj=inf.primal_obj,
+                            dual_obj=inf.dual_obj, step_size=inf.step_size,
+                            primal_weight=inf.primal_weight, scale_obj=inf.scale_obj,
+                            device_seconds=secs, team="batch", precision=int(precision))
+            sol = Solution(status=st, objective=float(inf.objective),
+                           iterations=int(inf.iterations), time=secs, method="pdlp[gpu,batch]")
+            if keep_outputs:
+                sol.x = xall[ox: ox + n].copy()
+                sol.reduced_costs = dall[ox: ox + n].copy()
+                sol.y = yall[oy: oy + m].copy()
+                ox += n
+                oy += m
+            sol.dual_bound = sol.objective
+            sol.work_units = float(inf.iterations)
+            sol.info = info
+            sols.append(sol.drop_objective_if_unsolved())
+    finally:
+        _lib.lib().sovlp_batch_destroy(h)
+    lat = np.array(lat)
+    return BatchResult(sols, best, int(g.value), db.count / best,
+                       float(np.percentile(lat, 50)), float(np.percentile(lat, 99)),
+                       int(precision), int(block))

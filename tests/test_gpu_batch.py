@@ -1,0 +1,72 @@
+"""Batched PDLP on the GPU: every LP of an uneven batch reaches its closed-form
+optimum, and an LP's result is identical alone and in the mix."""
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from sovlp.core.problem import Status
+from sovlp.lp.batch import solve_batch
+from sovlp.lp.pdlp import PDLPParams, solve_pdlp
+from sovlp.workloads import uneven_batch_member
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(count, base=2000):
+    return [uneven_batch_member(base + i) for i in range(count)]
+
+
+def test_fp64_batch_matches_closed_form_and_oracle():
+    print("[gpu] fp64 batch of 12 uneven LPs", flush=True)
+    kos = _batch(12)
+    par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6)
+    res = solve_batch([k.problem for k in kos], par, precision=64, block=256)
+    print(f"[gpu] batch done in {res.seconds:.3f}s, {res.graphs} graph replays", flush=True)
+    for ko, s in zip(kos, res.solutions):
+        assert s.status == Status.OPTIMAL, (ko.problem.name, s)
+        assert abs(s.objective - ko.f_star) <= 1e-5 * max(1.0, abs(ko.f_star))
+        rv, bv, _ = ko.problem.violation(s.x)
+        assert rv <= 1e-6 and bv <= 1e-12
+    # the oracle on two of them
+    for ko, s in list(zip(kos, res.solutions))[:2]:
+        r = oracle.solve(ko.problem, eps_abs=1e-6, eps_rel=1e-6)
+        assert r.status == 0
+        assert abs(s.objective - r.objective) <= 1e-5 * max(1.0, abs(r.objective))
+
+
+def test_result_is_identical_alone_and_in_the_mix():
+    print("[gpu] alone vs mixed", flush=True)
+    kos = _batch(6, base=3000)
+    par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6)
+    mixed = solve_batch([k.problem for k in kos], par, precision=64, block=256)
+    for i in (0, 3):
+        alone = solve_batch([kos[i].problem], par, precision=64, block=256)
+        a, b = alone.solutions[0], mixed.solutions[i]
+        assert a.iterations == b.iterations
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+        assert a.objective == b.objective
+    # the single-LP engine with one workgroup runs the same arithmetic
+    s = solve_pdlp(kos[0].problem, PDLPParams(device="gpu", team="block", eps_abs=1e-6, eps_rel=1e-6))
+    assert s.iterations == mixed.solutions[0].iterations
+    assert np.array_equal(s.x, mixed.solutions[0].x)
+
+
+def test_fp32_batch_reaches_loose_tolerance():
+    print("[gpu] fp32 batch", flush=True)
+    kos = _batch(8, base=4000)
+    par = PDLPParams(eps_abs=1e-4, eps_rel=1e-4, feas_cap=1e-3)
+    res = solve_batch([k.problem for k in kos], par, precision=32, block=256)
+    for ko, s in zip(kos, res.solutions):
+        assert s.status == Status.OPTIMAL, (ko.problem.name, s)
+        assert abs(s.objective - ko.f_star) <= 1e-3 * max(1.0, abs(ko.f_star))
+
+
+def test_small_workgroups_agree_with_large():
+    print("[gpu] block 64 vs 256", flush=True)
+    kos = _batch(4, base=5000)
+    par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6)
+    a = solve_batch([k.problem for k in kos], par, precision=64, block=64)
+    for ko, s in zip(kos, a.solutions):
+        assert s.status == Status.OPTIMAL
+        assert abs(s.objective - ko.f_star) <= 1e-5 * max(1.0, abs(ko.f_star))

@@ -1,0 +1,116 @@
+"""Host-side logic: matrix builder conventions, generators, boundary types, and the
+C ABI library's load and exports (no compute calls: this suite runs without a GPU)."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from sovlp import _lib
+from sovlp.core.problem import Solution, Status
+from sovlp.core.sparse import SparseMatrix
+from sovlp.workloads import (skewed_lp_degrees, small_lp, uneven_batch_member)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_triplets_sum_duplicates_and_drop_zeros():
+    # sparse.py:198-236: (0,0) listed twice is summed; an explicit zero disappears
+    A = SparseMatrix.from_triplets([0, 0, 1, 1], [0, 0, 1, 2], [1.0, 2.5, 0.0, 4.0], 2, 3)
+    assert A.nnz == 2
+    np.testing.assert_array_equal(A.to_dense(), [[3.5, 0, 0], [0, 0, 4.0]])
+    np.testing.assert_array_equal(A.cp, [0, 1, 1, 2])
+    np.testing.assert_array_equal(A.rp, [0, 1, 2])
+
+
+def test_csr_and_csc_hold_the_same_matrix():
+    rng = np.random.default_rng(0)
+    r, c = rng.integers(0, 30, 200), rng.integers(0, 40, 200)
+    A = SparseMatrix.from_triplets(r, c, rng.standard_normal(200), 30, 40)
+    D = np.zeros((30, 40))
+    for i in range(30):
+        D[i, A.ri[A.rp[i]:A.rp[i + 1]]] = A.rx[A.rp[i]:A.rp[i + 1]]
+    np.testing.assert_array_equal(D, A.to_dense())
+    # columns ascending inside every row
+    for i in range(30):
+        assert np.all(np.diff(A.ri[A.rp[i]:A.rp[i + 1]]) > 0)
+
+
+def test_status_codes_and_objective_drop():
+    assert [Status.OPTIMAL, Status.ITERATION_LIMIT, Status.TIME_LIMIT, Status.NUMERICAL] == [0, 4, 5, 8]
+    s = Solution(Status.NUMERICAL, objective=3.0, dual_bound=3.0).drop_objective_if_unsolved()
+    assert np.isnan(s.objective) and np.isnan(s.dual_bound)
+    s = Solution(Status.TIME_LIMIT, objective=3.0).drop_objective_if_unsolved()
+    assert s.objective == 3.0
+
+
+def test_small_lp_shape_and_optimum_certificate():
+    ko = small_lp(0)
+    p = ko.problem
+    assert (p.m, p.n) == (500, 2000) and p.nnz <= 10_000
+    eq, _, le, _, _ = p.row_types()
+    assert (eq, le) == (250, 250)
+    # KKT of (x*, y*) exactly: reduced costs have the right signs at the bounds
+    d = p.c - p.A.rmatvec(ko.y_star)
+    lo = ko.x_star == 0.0
+    hi = ko.x_star == 10.0
+    assert np.all(d[lo] > 0) and np.all(d[hi] < 0)
+    assert np.allclose(d[~lo & ~hi], 0.0, atol=1e-12)
+    assert p.violation(ko.x_star)[0] <= 1e-12
+
+
+def test_skewed_degrees_follow_the_rank_law():
+    deg = skewed_lp_degrees()
+    assert abs(int(deg.sum()) - 8_000_000) <= 5000
+    assert deg.max() == 5000 and deg.min() == 2
+    assert int((deg == 5000).sum()) > 100 and int((deg == 2).sum()) > 800_000
+
+
+def test_batch_member_ranges():
+    for s in range(20):
+        ko = uneven_batch_member(5000 + s)
+        p = ko.problem
+        assert 20 <= p.m <= 800 and 2 * p.m <= p.n <= 4 * p.m
+        ub = p.col_ub[0]
+        assert 5.0 <= ub <= 50.0 and np.all(p.col_ub == ub)
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "sovlp.h")).read()
+    declared = set(re.findall(r"\b(sovlp_[a-z0-9_]+)\s*\(", hdr))
+    L = C.CDLL(_lib.LIB_PATH)
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert declared == set(_lib.exported_symbols())
+    assert _lib.lib().sovlp_version() >= 100
+
+
+def test_tile_plan_host_function():
+    rng = np.random.default_rng(1)
+    lens = np.concatenate([rng.integers(0, 5, 3000), [5000, 3, 4096, 2049, 0, 7]])
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    out = np.zeros(lens.size + 1, dtype=np.int32)
+    nt = C.c_int32()
+    _lib.call("sovlp_plan_tiles", lens.size, ptr.ctypes.data, 2048, 256, out.ctypes.data, C.byref(nt))
+    t = out[: nt.value + 1]
+    assert t[0] == 0 and t[-1] == lens.size and np.all(np.diff(t) > 0)
+    for a, b in zip(t[:-1], t[1:]):
+        nz = ptr[b] - ptr[a]
+        if nz > 2048:
+            assert b - a == 1               # a long row is a tile of its own
+        else:
+            assert b - a <= 256
+    part = np.zeros(9, dtype=np.int32)
+    _lib.call("sovlp_partition_tiles", nt.value, t.ctypes.data, ptr.ctypes.data, 8, part.ctypes.data)
+    assert part[0] == 0 and part[-1] == nt.value and np.all(np.diff(part) >= 0)
+
+
+def test_backend_refuses_cpu_and_missing_gpu():
+    from sovlp.core import backend
+    with pytest.raises(ValueError):
+        backend.Backend("cpu")
+    if not backend.gpu_available():
+        with pytest.raises(RuntimeError):
+            backend.get_backend("gpu")

@@ -1,0 +1,150 @@
+"""The CPU oracle against what pins it.
+
+* Golden vectors written by running the reference's own ``scale_problem`` and
+  ``solve_pdlp(device="cpu")`` (``tests/golden/make_reference_vectors.py``).
+* Closed-form optima of the seeded known-optimum LPs.
+* An independent LP solver (SciPy's HiGHS) on small instances.
+* Planted mistakes in the oracle's inputs must be caught by the same checks.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from sovlp.core.problem import ObjSense, Problem
+from sovlp.core.sparse import SparseMatrix
+from sovlp.core.tolerances import INF
+from sovlp.workloads import known_optimum_lp, small_lp
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_pdlp.json")))
+
+
+def _problem(case):
+    A = SparseMatrix.from_triplets(np.array(case["rows"]), np.array(case["cols"]),
+                                   np.array(case["vals"], float), case["m"], case["n"])
+    return Problem(A=A, c=np.array(case["c"]), row_lb=np.array(case["rl"]),
+                   row_ub=np.array(case["ru"]), col_lb=np.array(case["lo"]),
+                   col_ub=np.array(case["hi"]), obj_offset=case["offset"],
+                   sense=ObjSense(case["sense"]), name=case["name"])
+
+
+@pytest.mark.parametrize("case", GOLD["cases"], ids=[c["name"] for c in GOLD["cases"]])
+def test_scaling_factors_equal_the_reference_bitwise(case):
+    p = _problem(case)
+    c = -p.c if case["sense"] == -1 else p.c
+    row, col, obj = oracle.scale(p, c)
+    e = case["expected"]
+    assert np.array_equal(row, np.array(e["scale_row"]))
+    assert np.array_equal(col, np.array(e["scale_col"]))
+    assert obj == e["scale_obj"]
+    # every factor is an exact power of two
+    for f in (row, col, np.array([obj])):
+        mant, _ = np.frexp(f)
+        assert np.all(mant == 0.5)
+
+
+@pytest.mark.parametrize("case", GOLD["cases"], ids=[c["name"] for c in GOLD["cases"]])
+def test_pdlp_agrees_with_reference_run(case):
+    p = _problem(case)
+    e = case["expected"]
+    r = oracle.solve(p, **case["params"])
+    assert r.status == e["status"]
+    tol = case["params"].get("eps_rel", 1e-8)
+    scale = max(1.0, abs(e["objective"]))
+    # both runs stop at relative KKT <= tol, so objectives agree to a few tol
+    assert abs(r.objective - e["objective"]) <= 10 * tol * scale
+    if "f_star" in case:
+        assert abs(r.objective - case["f_star"]) <= 10 * tol * max(1.0, abs(case["f_star"]))
+    # the trajectory agrees to rounding through the first check
+    h0, g0 = r.history[0] if r.history else None, e["history"][0] if e["history"] else None
+    if h0 is not None and g0 is not None:
+        assert h0[0] == g0[0]
+        for a, b in zip(h0[1:], g0[1:]):
+            assert abs(a - b) <= 1e-9 * max(1.0, abs(b))
+
+
+@pytest.mark.parametrize("name", ["toy_max", "testprob"])
+def test_trajectory_is_identical_on_reference_fixtures(name):
+    case = next(c for c in GOLD["cases"] if c["name"] == name)
+    e = case["expected"]
+    r = oracle.solve(_problem(case), **case["params"])
+    assert (r.iterations, r.restarts, r.rejected_steps) == \
+        (e["iterations"], e["restarts"], e["rejected_steps"])
+    np.testing.assert_allclose(r.x, e["x"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(r.y, e["y"], rtol=1e-8, atol=1e-8)
+
+
+def test_known_answers_of_the_reference_tests():
+    # toy max LP -> 11 at (3, 1) (tests/test_solvers.py:20-26, 50-55)
+    toy = next(c for c in GOLD["cases"] if c["name"] == "toy_max")
+    r = oracle.solve(_problem(toy))
+    assert r.status == 0 and abs(r.objective - 11.0) < 1e-6
+    np.testing.assert_allclose(r.x, [3.0, 1.0], atol=1e-5)
+    # testprob.mps -> 16 at (0, -1, 6) (tests/test_solvers.py:58-66)
+    tp = next(c for c in GOLD["cases"] if c["name"] == "testprob")
+    r = oracle.solve(_problem(tp))
+    assert r.status == 0 and abs(r.objective - 16.0) < 1e-7
+    np.testing.assert_allclose(r.x, [0.0, -1.0, 6.0], atol=1e-6)
+
+
+def test_small_lp_reaches_its_closed_form_optimum():
+    ko = small_lp(0)
+    r = oracle.solve(ko.problem, eps_abs=1e-8, eps_rel=1e-8)
+    assert r.status == 0
+    assert abs(r.objective - ko.f_star) <= 1e-7 * max(1.0, abs(ko.f_star))
+    assert abs(ko.f_star - ko.dual_star) <= 1e-9 * max(1.0, abs(ko.f_star))
+    rv, bv, _ = ko.problem.violation(r.x)
+    assert rv <= 1e-6 and bv == 0.0
+
+
+def test_maximise_and_minimise_agree():
+    toy = next(c for c in GOLD["cases"] if c["name"] == "toy_max")
+    p = _problem(toy)
+    q = p.copy()
+    q.c = -q.c
+    q.sense = ObjSense.MINIMISE
+    a, b = oracle.solve(p), oracle.solve(q)
+    assert abs(a.objective + b.objective) < 1e-5
+
+
+def test_highs_cross_check():
+    scipy_opt = pytest.importorskip("scipy.optimize")
+    from scipy.sparse import csc_matrix
+    for seed in range(3):
+        ko = known_optimum_lp(40, 100, 4, seed=100 + seed, m_eq=20)
+        p = ko.problem
+        A = csc_matrix((p.A.cx, p.A.ci, p.A.cp), shape=(p.m, p.n))
+        eq = p.row_lb > -INF
+        h = scipy_opt.linprog(p.c, A_ub=A[~eq], b_ub=p.row_ub[~eq], A_eq=A[eq],
+                              b_eq=p.row_ub[eq], bounds=list(zip(p.col_lb, p.col_ub)),
+                              method="highs")
+        r = oracle.solve(p, eps_abs=1e-8, eps_rel=1e-8)
+        assert h.status == 0 and r.status == 0
+        assert abs(h.fun - ko.f_star) <= 1e-9 * max(1.0, abs(ko.f_star))
+        assert abs(r.objective - h.fun) <= 1e-6 * max(1.0, abs(h.fun))
+
+
+def test_planted_mistakes_are_caught():
+    """A wrong sign on c or a dropped duplicate in A must move the optimum away
+    from f* far beyond the tolerances used above."""
+    ko = known_optimum_lp(40, 100, 4, seed=7, m_eq=20)
+    p = ko.problem.copy()
+    p.c = -p.c
+    r = oracle.solve(p, eps_abs=1e-8, eps_rel=1e-8, max_iter=20000)
+    assert abs(r.objective - ko.f_star) > 1e-3 * max(1.0, abs(ko.f_star))
+    # equality rows read as <= rows (a dropped lower bound)
+    q = ko.problem.copy()
+    q.row_lb[:] = -1e30
+    r = oracle.solve(q, eps_abs=1e-8, eps_rel=1e-8, max_iter=20000)
+    assert abs(r.objective - ko.f_star) > 1e-3 * max(1.0, abs(ko.f_star))
+    # a wrong index: every row index shifted by one
+    A = ko.problem.A
+    cols = np.repeat(np.arange(A.n), np.diff(A.cp))
+    shifted = SparseMatrix.from_triplets((A.ci + 1) % A.m, cols, A.cx, A.m, A.n)
+    q = ko.problem.copy()
+    q.A = shifted
+    r = oracle.solve(q, eps_abs=1e-8, eps_rel=1e-8, max_iter=20000)
+    assert r.status != 0 or abs(r.objective - ko.f_star) > 1e-3 * max(1.0, abs(ko.f_star))
