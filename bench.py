@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Throughput of batched PDLP on one MI355X: LPs/s solved to relative KKT 1e-6.
+
+Workload: 4096 independent synthetic LPs with known optima (seeds 1000..5095),
+m ~ log-uniform[20, 800] with 60 % equality rows, n = U[2,4] m, 3-8 nonzeros
+per column, values N(0,1), bounds [0, U(5,50)] per LP.  A step solves the
+whole batch once, from scratch (scaling, power iteration, PDHG to termination
+under the reference's _converged test with eps_abs = eps_rel = feas_cap =
+1e-6), with the inputs resident in HBM.
+
+Beside the headline number the JSON line carries:
+  plain_fp64    the same LPs one at a time through the single-LP engine in
+                FP64 (inputs resident, a sample), and the ratio
+  roofline      algorithmic bytes of the step kernel (phase counts x the byte
+                model of each phase) / its average launch time (HIP events on
+                the launch stream), against 8 TB/s
+  cpu_baseline  the FP64 C restatement of the reference solver on a bounded
+                sample of the batch, one LP per host thread
+  spmv          the tile-pass SpMV of A and A^T of the 200k x 1M skewed LP
+                (optional, --spmv)
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lps 4096]
+                       [--precision 64|32] [--block 256|64]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sovereign-moe_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = ("LPs/s at rel KKT≤1e-6: batched FP32+refine vs plain FP64 GPU PDLP; p50/p99")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lps", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--precision", type=int, default=64, choices=(64, 32))
+    ap.add_argument("--block", type=int, default=256, choices=(256, 64))
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--plain-sample", type=int, default=48,
+                    help="LPs solved one at a time for the FP64 single-LP rate (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="wall budget of the CPU baseline sample (0: skip)")
+    ap.add_argument("--spmv", action="store_true", help="also time the SpMV of the 200k x 1M LP")
+    return ap.parse_args()
+
+
+def plain_fp64_rate(problems, tol, sample):
+    """LPs/s of the single-LP FP64 engine, one LP at a time, inputs resident."""
+    import torch
+    from sovlp.lp.pdlp import DeviceLP, Engine, PDLPParams
+    par = PDLPParams(eps_abs=tol, eps_rel=tol, feas_cap=tol, device="gpu", precision=64)
+    total, iters, solved = 0.0, 0, 0
+    for p in problems[:sample]:
+        dlp = DeviceLP(p, 64)
+        eng = Engine(dlp, par, history_cap=1)
+        try:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            while True:
+                eng.advance(par.chunk_iters, par.time_limit)
+                inf = eng.info()
+                if inf.stage == 3:
+                    break
+            total += time.perf_counter() - t0
+            iters += int(inf.iterations)
+            solved += int(inf.status == 0)
+        finally:
+            eng.close()
+    return dict(lps=sample, seconds=total, value=sample / total, unit="LPs/s",
+                optimal=solved, mean_iterations=iters / max(sample, 1))
+
+
+def cpu_baseline(problems, tol, budget_s):
+    """The FP64 C restatement of the reference solver, one LP per host thread,
+    over as many LPs of the batch as finish within about ``budget_s``."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle
+    oracle.lib()
+    cores = len(os.sched_getaffinity(0))
+    cores = max(1, min(cores, 16))
+    done = []
+    t0 = time.perf_counter()
+    deadline = t0 + budget_s
+
+    def work(i):
+        if time.perf_counter() > deadline:
+            return None
+        r = oracle.solve(problems[i], eps_abs=tol, eps_rel=tol, feas_cap=tol, threads=1,
+                         hist_cap=1)
+        return r.status
+
+    with ThreadPoolExecutor(cores) as ex:
+        for st in ex.map(work, range(len(problems))):
+            if st is not None:
+                done.append(st)
+    el = time.perf_counter() - t0
+    return dict(value=len(done) / el, unit="LPs/s", cores=cores, kind="port",
+                sample=f"first {len(done)} LPs of the batch, FP64 C restatement of the "
+                       f"reference PDLP, one LP per thread, {el:.1f} s wall",
+                optimal=int(sum(1 for s in done if s == 0)))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every LP of the batch lives on one device and "
+              "this build runs no multi-GPU path (--gpus must be 1).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    from sovlp.lp.batch import BatchEngine
+    from sovlp.lp.pdlp import PDLPParams
+    from sovlp.lp.traffic import bytes_of_counts
+    from sovlp.workloads import uneven_batch
+
+    t = time.perf_counter()
+    kos = uneven_batch(a.lps, base_seed=a.seed)
+    problems = [k.problem for k in kos]
+    log(f"generated {a.lps} LPs in {time.perf_counter() - t:.1f} s "
+        f"(sum nnz {sum(p.A.nnz for p in problems)})")
+    par = PDLPParams(eps_abs=a.tol, eps_rel=a.tol, feas_cap=a.tol, device="gpu")
+    t = time.perf_counter()
+    eng = BatchEngine(problems, par, precision=a.precision, block=a.block)
+    log(f"batch laid out on the device in {time.perf_counter() - t:.1f} s")
+    for w in range(a.warmup):
+        g = eng.run()
+        log(f"warmup {w}: {g} graph replays")
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    graphs = []
+    for k in range(a.steps):
+        graphs.append(eng.run())
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = ev0.elapsed_time(ev1)
+    log(f"timed {a.steps} steps: {wall:.3f} s wall, {dev_ms:.1f} ms on the stream, "
+        f"graphs {graphs}")
+
+    sols, lat = eng.solutions(keep_outputs=True)
+    opt = sum(1 for s in sols if int(s.status) == 0)
+    rel = [abs(s.objective - k.f_star) / max(1.0, abs(k.f_star))
+           for s, k in zip(sols, kos) if s.objective is not None]
+    counts = eng.phase_counts()
+    step_bytes = bytes_of_counts(counts, eng.sizes(), a.precision)
+    launches = sum(graphs)            # one run-kernel launch per graph replay
+    launch_ms = dev_ms / launches
+    achieved = step_bytes / graphs[-1] / (launch_ms * 1e-3) / 1e9
+    kernel = (f"pdlp_run_kernel<{'double' if a.precision == 64 else 'float'}, {a.block}, "
+              f"{2048 if a.block == 256 else 512}>")
+    eng.close()
+
+    out = {
+        "metric": METRIC,
+        "value": a.lps * a.steps / wall,
+        "unit": "LPs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == 64 else "f32 iterates, f64 reductions",
+        "data": "synthetic",
+        "config": {"workload": f"{a.lps} uneven LPs with known optima, m log-uniform[20,800] "
+                               f"(60% eq), n=U[2,4]m, 3-8 nnz/col, bounds [0,U(5,50)], "
+                               f"seeds {a.seed}..{a.seed + a.lps - 1}",
+                   "lps": a.lps, "tolerance": a.tol, "precision": a.precision,
+                   "block": a.block, "parallelism": "one workgroup per LP"},
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "optimal": opt,
+        "max_rel_obj_err_vs_closed_form": float(max(rel)) if rel else None,
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "algorithmic_bytes_per_launch": step_bytes / graphs[-1],
+                     "avg_launch_us": launch_ms * 1e3, "launches": launches},
+    }
+    if a.plain_sample > 0:
+        t = time.perf_counter()
+        pl = plain_fp64_rate(problems, a.tol, min(a.plain_sample, a.lps))
+        log(f"plain fp64 sample: {pl} ({time.perf_counter() - t:.1f} s)")
+        out["plain_fp64"] = pl
+        out["speedup_vs_plain_fp64"] = out["value"] / pl["value"]
+    if a.cpu_seconds > 0:
+        cb = cpu_baseline(problems, a.tol, a.cpu_seconds)
+        log(f"cpu baseline: {cb}")
+        out["cpu_baseline"] = cb
+    if a.spmv:
+        from sovlp.spmv_bench import skewed_roofline
+        res, gen = skewed_roofline()
+        out["spmv"] = res
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

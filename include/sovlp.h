@@ -116,13 +116,26 @@ int sovlp_spmv_tiled_f32(int rows, const int32_t *ptr32, const int32_t *idx, con
 
 /* ---- PDLP engine: src/sovopt/lp/pdlp.py:238-494 ---------------------- */
 
-/* PDLPParams, pdlp.py:79-116 (same fields and defaults). */
+/* PDLPParams, pdlp.py:79-116 (same fields and defaults), then the
+ * refinement settings of the FP32 engines (no reference counterpart):
+ * refine_rounds  0 = none (required at precision 64), else the most solves
+ *                (the plain one included) of an outer iterative-refinement
+ *                loop, at most 16.  Round r > 0 solves min c'x^ subject to
+ *                rl <= A x^ <= ru and lo <= x^ <= hi shifted by the FP64
+ *                accumulated X (and A X) and scaled by a power of two Delta;
+ *                X += x^ / Delta.  The dual carries over between rounds.
+ * refine_eps     relative tolerance (eps_abs = eps_rel) of every round
+ * refine_tol     the loop also stops once the primal residual of X is
+ *                <= refine_tol (1 + ||finite row bounds||_inf), after
+ *                _converged (pdlp.py:523-530) holds for X on the original. */
 typedef struct {
   double eps_abs, eps_rel, feas_cap;
   int64_t max_iter;
   double time_limit;
   int32_t check_every, stepsize_every;
   double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
+  int32_t refine_rounds, refine_pad;
+  double refine_eps, refine_tol;
 } sovlp_params;
 
 /* One LP in minimisation form (costs and offset already negated for a
@@ -153,6 +166,9 @@ typedef struct {
   double primal_res, dual_res, gap, primal_obj, dual_obj;
   double step_size, primal_weight, objective, scale_obj;
   int64_t device_ticks;     /* device clock ticks spent (sovlp_clock_rate) */
+  int32_t refine_rows;      /* rows of the refinement record */
+  int32_t refine_round;     /* last refinement round started (0: none) */
+  double refine_scale;      /* Delta of that round */
 } sovlp_info;
 
 /* Workspace bytes for one LP at `precision` 64 (FP64 iterates) or 32 (FP32
@@ -165,9 +181,11 @@ int sovlp_pdlp_workspace_bytes(const sovlp_lp *lp, int precision, int64_t histor
 typedef struct sovlp_pdlp sovlp_pdlp;
 
 /* Prepares a solve in the caller's workspace (>= workspace_bytes) and records
- * 64 phases of it into a hipGraph.  `team` 0 = choose, 1 = one workgroup per
- * phase, 2 = W workgroups per phase (W from the tile count) with a one-
- * workgroup reduction launch after each phase.  Synchronises. */
+ * one replay of it into a hipGraph.  `team` 0 = choose, 1 = one workgroup
+ * (a replay is one launch running up to 128 phases of the solve back to
+ * back), 2 = W workgroups per phase (W from the tile count; a replay is 64
+ * phase launches, each followed by a one-workgroup reduction launch).
+ * Synchronises. */
 int sovlp_pdlp_create(const sovlp_lp *lp, const sovlp_params *params, int precision,
                       void *workspace, int64_t workspace_bytes, int64_t history_cap, int team,
                       sovlp_stream_t stream, sovlp_pdlp **out);
@@ -185,18 +203,36 @@ int sovlp_pdlp_info(sovlp_pdlp *h, sovlp_info *info, sovlp_stream_t stream);
  * and of the history rows (k, pres, dres, gap, pobj), valid once stage == 3. */
 int sovlp_pdlp_outputs(sovlp_pdlp *h, double **x, double **y, double **d, double **history);
 
+/* Device pointer of the refinement record: refine_rows rows of (iterations
+ * so far, primal residual, dual residual, gap, objective of the minimisation
+ * form) of X on the original problem, one per finished round. */
+int sovlp_pdlp_refinement(sovlp_pdlp *h, double **rows);
+
 /* Workgroups per phase chosen by sovlp_pdlp_create. */
 int sovlp_pdlp_width(sovlp_pdlp *h, int *workgroups);
 
 int sovlp_pdlp_destroy(sovlp_pdlp *h);
 
+/* ---- phase accounting (no reference counterpart) ----------------------
+ * The engines count, per LP, how often each phase of the solve ran.  Slot
+ * `s` < sovlp_phase_slots() is named by sovlp_phase_name(s) (INIT, ...,
+ * B_FUSED, CHECK, ..., plus B_ADAPT_NEXT / B_FUSED_NEXT: the B phases that
+ * also formed the next primal step).  With the byte model of each phase
+ * this gives the algorithmic HBM bytes of a solve. */
+int sovlp_phase_slots(int *slots);
+const char *sovlp_phase_name(int slot);
+
+/* counts[sovlp_phase_slots()] of the engine's LP; synchronises. */
+int sovlp_pdlp_phase_counts(sovlp_pdlp *h, int64_t *counts, sovlp_stream_t stream);
+
 /* ---- batched PDLP: many independent LPs in one persistent launch ------
  * No reference counterpart: the reference solves one LP per call
  * (pdlp.py:238); its only batched PDHG shares one A across B&B nodes
  * (src/sovopt/mip/bnr.py:176-301).  Here each LP keeps its own matrix,
- * scaling, step size, restarts and termination; one workgroup solves one LP
- * at a time and pulls the next from a device queue (largest first), so the
- * arithmetic of an LP does not depend on the other LPs in the batch.
+ * scaling, step size, restarts and termination.  Every launch advances every
+ * unfinished LP by up to 128 phases, one workgroup per LP, and the launch is
+ * replayed from a hipGraph, so the arithmetic of an LP does not depend on the
+ * other LPs in the batch.
  * Tile plans must use sovlp_batch_geometry's tile size and row count. */
 typedef struct sovlp_batch sovlp_batch;
 
@@ -208,9 +244,10 @@ int sovlp_batch_geometry(int precision, int block, int *tile_rows, int *tile_nnz
 int sovlp_batch_workspace_bytes(int count, const sovlp_lp *lps, int precision,
                                 int64_t history_cap, int64_t *bytes);
 
-/* Lays out the batch in the workspace and records 64 phase launches (one
- * workgroup of `block` threads per LP each) into a hipGraph.  `block` is 64
- * or 256.  Synchronises. */
+/* Lays out the batch in the workspace and records one replay into a
+ * hipGraph: one launch, one workgroup of `block` threads per LP, each running
+ * up to 128 phases of its LP back to back.  `block` is 64 or 256.
+ * Synchronises. */
 int sovlp_batch_create(int count, const sovlp_lp *lps, const sovlp_params *params,
                        int precision, int block, void *workspace, int64_t workspace_bytes,
                        int64_t history_cap, sovlp_stream_t stream, sovlp_batch **out);
@@ -226,6 +263,12 @@ int sovlp_batch_infos(sovlp_batch *h, sovlp_info *infos, sovlp_stream_t stream);
 
 /* Device pointers of LP `index`'s report (FP64, original units). */
 int sovlp_batch_outputs(sovlp_batch *h, int index, double **x, double **y, double **d);
+
+/* Device pointer of LP `index`'s refinement record (as sovlp_pdlp_refinement). */
+int sovlp_batch_refinement(sovlp_batch *h, int index, double **rows);
+
+/* counts[count * sovlp_phase_slots()], LP-major; synchronises. */
+int sovlp_batch_phase_counts(sovlp_batch *h, int64_t *counts, sovlp_stream_t stream);
 
 int sovlp_batch_destroy(sovlp_batch *h);
 

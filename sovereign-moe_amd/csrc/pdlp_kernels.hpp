@@ -1,7 +1,9 @@
 // pdlp_kernels.hpp -- the PDLP launch kernels (device code only).
 //
-// Each launch runs one phase (pdlp_solver.hpp) of each LP it serves:
-//   pdlp_step_kernel     one workgroup per LP; reduces and takes the transition
+// Each launch runs phases (pdlp_solver.hpp) of the LPs it serves:
+//   pdlp_run_kernel      one workgroup per LP runs up to `max_phases` phases of
+//                        its LP back to back (state in LDS, workgroup barriers
+//                        between phases; no workgroup waits for another)
 //   pdlp_phase_kernel    W workgroups share one LP's phase, write partial sums
 //   pdlp_decide_kernel   one workgroup reduces the W partials (fixed order) and
 //                        takes the transition
@@ -49,8 +51,11 @@ __device__ __forceinline__ void take_transition(const LpDev<T>& L, const LpState
     ss.tick0 = (long long)wall_clock64();
     ss.ticks_left = ctl->ticks;
   }
+  L.pcount[ph] += 1;
+  if ((ph == PH_B_ADAPT || ph == PH_B_FUSED) && s.next_mode != NEXT_NONE)
+    L.pcount[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
   Transition tr{ctl->P, L.offset0, L.flip};
-  ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap);
+  ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap, L.rhist);
   if (ss.phase == PH_DONE) {
     ss.ticks = (long long)wall_clock64() - ss.tick0;
     atomicAdd(done, 1u);
@@ -58,24 +63,65 @@ __device__ __forceinline__ void take_transition(const LpDev<T>& L, const LpState
   *L.st = ss;
 }
 
-// One workgroup per LP: run the LP's current phase, reduce, take the transition.
+// Thread 0: apply the transition of phase `ph` to the state held in LDS.
+template <class T>
+__device__ __forceinline__ void transition_in_lds(const LpDev<T>& L, int ph, const double* t,
+                                                  const StepCtl* ctl, unsigned* done,
+                                                  LpState& ss) {
+  if (ph == PH_INIT) {
+    ss.tick0 = (long long)wall_clock64();
+    ss.ticks_left = ctl->ticks;
+  }
+  L.pcount[ph] += 1;
+  if ((ph == PH_B_ADAPT || ph == PH_B_FUSED) && ss.next_mode != NEXT_NONE)
+    L.pcount[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
+  Transition tr{ctl->P, L.offset0, L.flip};
+  ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap, L.rhist);
+  if (ss.phase == PH_DONE) {
+    ss.ticks = (long long)wall_clock64() - ss.tick0;
+    atomicAdd(done, 1u);
+  }
+}
+
+// One workgroup per LP (blockIdx.x): up to `max_phases` phases of the LP in a
+// row.  Phase k+1 reads what phase k wrote to global memory; the workgroup
+// barrier between them orders those accesses (all of a workgroup's waves share
+// one CU).  The loop is bounded, and the state goes back to HBM at the end.
 template <class T, int BLOCK, int TNNZ>
 __global__ void __launch_bounds__(BLOCK)
-    pdlp_step_kernel(const LpDev<T>* __restrict__ lps, const StepCtl* __restrict__ ctl,
-                     unsigned* done) {
+    pdlp_run_kernel(const LpDev<T>* __restrict__ lps, const StepCtl* __restrict__ ctl,
+                    unsigned* done, int max_phases) {
   __shared__ __align__(16) double lds_raw[2 * TNNZ];
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   __shared__ LpState ss;
   const LpDev<T>& L = lps[blockIdx.x];
-  const LpState s = *L.st;
-  const int ph = uniform(s.phase);
-  if (ph == PH_DONE) return;
-  double part[kMaxRed];
-  init_part(ph, part);
-  PhaseWork<T, BLOCK, TNNZ> W(L, s, reinterpret_cast<T*>(lds_raw), 0, 1);
-  W.run(ph, part);
-  reduce_phase_to_lds<BLOCK>(ph, part, red);
-  if (threadIdx.x == 0) take_transition(L, s, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
+  if (threadIdx.x == 0) ss = *L.st;
+  __syncthreads();
+  if (uniform(ss.phase) == PH_DONE) return;
+  for (int it = 0; it < max_phases; ++it) {
+    const int ph = uniform(ss.phase);
+    if (ph == PH_DONE) break;
+    // the fused A phase reduces nothing, and the fused B phase only ||dx||^2
+    // for a step-size trial that follows it.  Read here, while no thread
+    // writes the state: thread 0 changes it right after the phase.
+    const bool reduce =
+        !(ph == PH_A_FUSED || (ph == PH_B_FUSED && uniform(ss.next_mode) != NEXT_TRIAL));
+    double part[kMaxRed];
+    init_part(ph, part);
+    PhaseWork<T, BLOCK, TNNZ> W(L, ss, reinterpret_cast<T*>(lds_raw), 0, 1);
+    W.run(ph, part);
+    if (reduce) reduce_phase_to_lds<BLOCK>(ph, part, red);
+    if (threadIdx.x == 0) {
+      if (ph == PH_A_FUSED) {
+        L.pcount[PH_A_FUSED] += 1;
+        ss.phase = PH_B_FUSED;
+      } else {
+        transition_in_lds(L, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *L.st = ss;
 }
 
 // W workgroups share one LP's phase; each writes its partial sums.
@@ -135,6 +181,7 @@ __global__ void state_reset_kernel(const LpDev<T>* __restrict__ lps, int count, 
   s.phase = PH_INIT;
   s.status = ST_RUNNING;
   *lps[i].st = s;
+  for (int k = 0; k < kPhaseSlots; ++k) lps[i].pcount[k] = 0;
 }
 
 

@@ -51,15 +51,27 @@ struct Params {              // same layout as sovlp_params in include/sovlp.h
   double time_limit;
   int check_every, stepsize_every;
   double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
+  int refine_rounds, refine_pad;   // 0: no refinement (FP64 engines always 0)
+  double refine_eps;         // relative KKT tolerance of each refinement round
+  double refine_tol;         // primal residual target: refine_tol (1 + ||b||_inf)
 };
+
+constexpr int kMaxRounds = 16;   // refinement rounds recorded per LP
 
 enum Phase : int {
   PH_INIT = 0, PH_RUIZ_MAX, PH_RUIZ_APPLY, PH_PC_SUM, PH_PC_APPLY, PH_SCALE, PH_AX0, PH_POW_A,
   PH_POW_T, PH_KKT0, PH_P_TRIAL, PH_A_TRIAL, PH_B_ADAPT, PH_A_FUSED, PH_B_FUSED, PH_CHECK,
-  PH_RESTART, PH_COPY_AVG, PH_FINAL, PH_FINAL_D, PH_DONE
+  PH_RESTART, PH_COPY_AVG, PH_FINAL, PH_FINAL_D, PH_REF_X, PH_REF_KKT, PH_REF_SHIFT, PH_DONE
 };
 
 enum NextMode : int { NEXT_NONE = 0, NEXT_FUSED = 1, NEXT_TRIAL = 2 };
+
+// Per-LP phase counters: slot ph counts executions of phase ph; the two extra
+// slots count the B phases that also formed the next primal step (they move
+// more bytes).  The host turns the counts into algorithmic bytes.
+constexpr int kSlotBAdaptNext = PH_DONE + 1;
+constexpr int kSlotBFusedNext = PH_DONE + 2;
+constexpr int kPhaseSlots = 28;
 
 struct LpState {             // everything the loop carries between phases
   double eta, omega, sum_eta, mu_restart, mu_last, mu_cand, anorm, sobj, ptol, dtol;
@@ -70,7 +82,18 @@ struct LpState {             // everything the loop carries between phases
   long long tick0;           // device clock when the solve started (phase INIT)
   long long ticks;           // device clock ticks used (set when the solve ends)
   long long ticks_left;      // time budget in ticks
-  int phase, status, sweep, pit, trial, primal_ready, time_flag, use_avg, next_mode, pad;
+  // iterative refinement (FP32 iterates): X accumulates in FP64 (xacc); each
+  // round solves min c'x^ over bounds shifted by X and scaled by ref_scale
+  double ref_scale;          // Delta of the current round (power of two)
+  double rhs0;               // ||finite row bounds||_inf of the original problem
+  double cnorm;              // ||c'||_2 of the scaled objective
+  double ptol0, dtol0;       // _converged tolerances of the original problem
+  long long k_done;          // iterations of finished rounds
+  long long kmax;            // iteration limit of the current round
+  int phase, status, sweep, pit, trial, primal_ready, time_flag, use_avg, next_mode;
+  int round;                 // refinement round (0: the plain solve)
+  int absorbed;              // xacc already holds the current round's x
+  int ref_rows;              // rows written to rhist
 };
 
 template <class T>
@@ -97,7 +120,14 @@ struct LpDev {
   double *xo, *yo, *dout;
   double* hist;
   long long hist_cap;
+  // bounds of the problem the rounds solve (FP64, unscaled): the original
+  // arrays when there is no refinement, the shifted ones otherwise
+  const double *crl, *cru, *clo, *chi;
+  double *wrl, *wru, *wlo, *whi;   // writable copies (refinement only)
+  double *xacc, *axacc;      // X and A X (refinement only; null otherwise)
+  double* rhist;             // [kMaxRounds][5] per round: k, pres, dres, gap, pobj
   LpState* st;
+  long long* pcount;         // [kPhaseSlots] phase counters
   double* partials;          // [W][kMaxRed] per-workgroup partial sums (W-way phases)
 };
 
@@ -129,6 +159,8 @@ enum { PC = 0, DC, CXC, SUC, BNC, PA, DA, CXA, SUA, BNA, PUC, DUC, CXUC, SUUC, B
 __host__ __device__ constexpr uint32_t phase_maxmask(int ph) {
   return ph == PH_RUIZ_MAX ? 1u : ph == PH_PC_APPLY ? 3u : ph == PH_SCALE ? ((1u << 3) | (1u << 4))
        : ph == PH_KKT0 ? (1u | 2u | 32u)
+       : ph == PH_REF_KKT ? (1u | 2u | 32u)
+       : ph == PH_REF_SHIFT ? 1u
        : ph == PH_CHECK ? ((1u << PC) | (1u << DC) | (1u << PA) | (1u << DA) | (1u << PUC) |
                            (1u << DUC) | (1u << PUA) | (1u << DUA) | (1u << TFLAG))
        : 0u;
@@ -266,6 +298,11 @@ struct PhaseWork {
           const double lo0 = l.lo0[j], hi0 = l.hi0[j];
           const T lo = (T)(lo0 <= -kInf ? -kInf : (lo0 >= kInf ? kInf : lo0 / cf));
           const T hi = (T)(hi0 <= -kInf ? -kInf : (hi0 >= kInf ? kInf : hi0 / cf));
+          if (l.xacc) {     // round 0 solves the original problem; X = 0
+            l.wlo[j] = lo0;
+            l.whi[j] = hi0;
+            l.xacc[j] = 0.0;
+          }
           l.c[j] = cj;
           l.lo[j] = lo;
           l.hi[j] = hi;
@@ -283,6 +320,10 @@ struct PhaseWork {
           const double rl0 = l.rl0[i], ru0 = l.ru0[i];
           const T rl = (T)(rl0 <= -kInf ? -kInf : (rl0 >= kInf ? kInf : rl0 * rf));
           const T ru = (T)(ru0 <= -kInf ? -kInf : (ru0 >= kInf ? kInf : ru0 * rf));
+          if (l.xacc) {
+            l.wrl[i] = rl0;
+            l.wru[i] = ru0;
+          }
           l.rl[i] = rl;
           l.ru[i] = ru;
           l.y[i] = T(0);
@@ -352,37 +393,40 @@ struct PhaseWork {
 
       case PH_A_TRIAL: {    // adx = A dx; y' = prox(y + sigma (ax + 2 adx)); dy.adx, dy.dy
         const T sg = (T)(s.eta * s.omega);
+        struct R { T y, ax, rl, ru; };
         auto prod = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * l.dx[j]; };
-        auto epi = [&](int i, const T (&sm)[1]) {
+        auto pre = [&](int i) { return R{l.y[i], l.ax[i], l.rl[i], l.ru[i]}; };
+        auto epi = [&](int i, const T (&sm)[1], const R& r) {
           const T a = sm[0];
           l.adx[i] = a;
-          const T yo = l.y[i];
-          const T w = yo + sg * (l.ax[i] + T(2) * a);
-          const T pr = clampT<T>(w / sg, l.rl[i], l.ru[i]);
+          const T w = r.y + sg * (r.ax + T(2) * a);
+          const T pr = clampT<T>(w / sg, r.rl, r.ru);
           const T yv = w - sg * pr;
           l.yn[i] = yv;
-          const T d = yv - yo;
+          const T d = yv - r.y;
           l.dy[i] = d;
           part[0] += (double)d * (double)a;
           part[1] += (double)d * (double)d;
         };
-        tile_pass<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, epi, lds);
+        tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
       } break;
 
       case PH_B_ADAPT: {    // accepted trial
         const T et = (T)s.eta, tt = (T)(s.eta / s.omega);
         const int nm = s.next_mode;
         double dxdx = 0.0;
+        struct R { T xn, aty, sumx; };
         auto prod = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * l.dy[i]; };
-        auto epi = [&](int j, const T (&sm)[1]) {
-          const T xj = l.xn[j];
+        auto pre = [&](int j) { return R{l.xn[j], l.aty[j], l.sumx[j]}; };
+        auto epi = [&](int j, const T (&sm)[1], const R& r) {
+          const T xj = r.xn;
           l.x[j] = xj;
-          const T aty = l.aty[j] + sm[0];
+          const T aty = r.aty + sm[0];
           l.aty[j] = aty;
-          l.sumx[j] += et * xj;
+          l.sumx[j] = r.sumx + et * xj;
           next_primal(j, xj, aty, nm, tt, dxdx);
         };
-        tile_pass<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, epi, lds);
+        tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
         for (long long i = i0 + tid; i < i1; i += BLOCK) {
           const T yv = l.yn[i];
           l.y[i] = yv;
@@ -394,35 +438,55 @@ struct PhaseWork {
 
       case PH_A_FUSED: {    // fused dual step (backend.py:260-279) and sum_y += eta y
         const T sg = (T)(s.eta * s.omega), et = (T)s.eta;
+        struct R { T ax, y, rl, ru, sumy; };
         auto prod = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * l.dx[j]; };
-        auto epi = [&](int i, const T (&sm)[1]) {
+        auto pre = [&](int i) { return R{l.ax[i], l.y[i], l.rl[i], l.ru[i], l.sumy[i]}; };
+        auto epi = [&](int i, const T (&sm)[1], const R& r) {
           const T a = sm[0];
-          const T axn = l.ax[i] + a;
+          const T axn = r.ax + a;
           l.ax[i] = axn;
-          const T yo = l.y[i];
-          const T w = yo + sg * (axn + a);
-          const T pr = clampT<T>(w / sg, l.rl[i], l.ru[i]);
+          const T w = r.y + sg * (axn + a);
+          const T pr = clampT<T>(w / sg, r.rl, r.ru);
           const T yv = w - sg * pr;
-          l.dy[i] = yv - yo;
+          l.dy[i] = yv - r.y;
           l.y[i] = yv;
-          l.sumy[i] += et * yv;
+          l.sumy[i] = r.sumy + et * yv;
         };
-        tile_pass<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, epi, lds);
+        tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
       } break;
 
       case PH_B_FUSED: {    // aty += A^T dy; sum_x += eta x (backend.py:282-291); next primal
         const T et = (T)s.eta, tt = (T)(s.eta / s.omega);
         const int nm = s.next_mode;
         double dxdx = 0.0;
+        struct R { T aty, x, sumx, c, lo, hi; };
         auto prod = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * l.dy[i]; };
-        auto epi = [&](int j, const T (&sm)[1]) {
-          const T aty = l.aty[j] + sm[0];
-          l.aty[j] = aty;
-          const T xj = l.x[j];
-          l.sumx[j] += et * xj;
-          next_primal(j, xj, aty, nm, tt, dxdx);
+        auto pre = [&](int j) {
+          R r{l.aty[j], l.x[j], l.sumx[j], T(0), T(0), T(0)};
+          if (nm != NEXT_NONE) {
+            r.c = l.c[j];
+            r.lo = l.lo[j];
+            r.hi = l.hi[j];
+          }
+          return r;
         };
-        tile_pass<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, epi, lds);
+        auto epi = [&](int j, const T (&sm)[1], const R& r) {
+          const T aty = r.aty + sm[0];
+          l.aty[j] = aty;
+          const T xj = r.x;
+          l.sumx[j] = r.sumx + et * xj;
+          if (nm == NEXT_NONE) return;
+          const T v = clampT<T>(xj - tt * (r.c + aty), r.lo, r.hi);
+          const T d = v - xj;
+          l.dx[j] = d;
+          if (nm == NEXT_FUSED) {
+            l.x[j] = v;
+          } else {
+            l.xn[j] = v;
+            dxdx += (double)d * (double)d;
+          }
+        };
+        tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
         part[0] += dxdx;
       } break;
 
@@ -458,8 +522,12 @@ struct PhaseWork {
       } break;
 
       case PH_FINAL: {      // x = clip(x col), y = -(y row / s); objective partial
+        const double rs = s.ref_scale;
         for (long long j = j0 + tid; j < j1; j += BLOCK) {
           double v = (double)l.x[j] * l.colf[j];
+          if (l.xacc) {     // X plus the current round's correction, if not yet in X
+            v = l.xacc[j] + (s.absorbed ? 0.0 : fmin(fmax(v, l.clo[j]), l.chi[j]) / rs);
+          }
           v = fmin(fmax(v, l.lo0[j]), l.hi0[j]);
           l.xo[j] = v;
           part[0] += l.c0[j] * v;
@@ -479,6 +547,78 @@ struct PhaseWork {
         tile_pass<BLOCK, TNNZ, 1, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prod, epi, ldsd);
       } break;
 
+      case PH_REF_X: {      // X += clip(x col, lo_r, hi_r) / Delta
+        const double rs = s.ref_scale;
+        for (long long j = j0 + tid; j < j1; j += BLOCK) {
+          const double v = fmin(fmax((double)l.x[j] * l.colf[j], l.clo[j]), l.chi[j]);
+          l.xacc[j] += v / rs;
+        }
+      } break;
+
+      case PH_REF_KKT: {    // FP64 KKT of (X, y) on the original problem; A X kept
+        const double so = s.sobj;
+        struct RR { double rl, ru, rf, y; };
+        auto prodA = [&](int q, int j, double (&p)[1]) { p[0] = l.av[q] * l.xacc[j]; };
+        auto preA = [&](int i) { return RR{l.rl0[i], l.ru0[i], l.rowf[i], (double)l.y[i]}; };
+        auto epiA = [&](int i, const double (&sm)[1], const RR& r) {
+          const double a = sm[0];
+          l.axacc[i] = a;
+          const double yu = (r.y * r.rf) / so;
+          part[0] = nanmax(part[0], fabs(fmax(r.rl - a, 0.0) + fmax(a - r.ru, 0.0)));
+          part[3] += bsel_row(yu, r.rl, r.ru) * yu;
+        };
+        tile_pass_pf<BLOCK, TNNZ, 1, false, double>(l.ap, l.ai, l.tilesA, a0, a1, prodA, preA,
+                                                    epiA, ldsd);
+        struct RC { double c, lo, hi, x; };
+        auto prodT = [&](int q, int i, double (&p)[1]) {
+          p[0] = l.tv[q] * (((double)l.y[i] * l.rowf[i]) / so);
+        };
+        auto preT = [&](int j) { return RC{l.c0[j], l.lo0[j], l.hi0[j], l.xacc[j]}; };
+        auto epiT = [&](int j, const double (&sm)[1], const RC& r) {
+          const double du = r.c + sm[0];
+          part[0] = nanmax(part[0], fabs(fmax(r.lo - r.x, 0.0) + fmax(r.x - r.hi, 0.0)));
+          part[1] = nanmax(part[1], fabs(dres_of(du, r.lo, r.hi)));
+          part[2] += r.c * r.x;
+          part[4] += bsel_col(du, r.lo, r.hi) * du;
+        };
+        tile_pass_pf<BLOCK, TNNZ, 1, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prodT, preT,
+                                                    epiT, ldsd);
+        part[5] = clock_flag();
+      } break;
+
+      case PH_REF_SHIFT: {  // next round: bounds shifted by X (and A X), scaled by Delta
+        const double rs = s.ref_scale;
+        for (long long j = j0 + tid; j < j1; j += BLOCK) {
+          const double X = l.xacc[j], lo0 = l.lo0[j], hi0 = l.hi0[j], cf = l.colf[j];
+          const double lo = lo0 > -kInf ? (lo0 - X) * rs : -kInf;
+          const double hi = hi0 < kInf ? (hi0 - X) * rs : kInf;
+          l.wlo[j] = lo;
+          l.whi[j] = hi;
+          const T los = (T)(lo > -kInf ? lo / cf : -kInf), his = (T)(hi < kInf ? hi / cf : kInf);
+          l.lo[j] = los;
+          l.hi[j] = his;
+          const T x0 = clampT<T>(T(0), los, his);
+          l.x[j] = x0;
+          l.xr[j] = x0;
+          l.sumx[j] = T(0);
+        }
+        for (long long i = i0 + tid; i < i1; i += BLOCK) {
+          const double a = l.axacc[i], rl0 = l.rl0[i], ru0 = l.ru0[i], rf = l.rowf[i];
+          const double rl = rl0 > -kInf ? (rl0 - a) * rs : -kInf;
+          const double ru = ru0 < kInf ? (ru0 - a) * rs : kInf;
+          l.wrl[i] = rl;
+          l.wru[i] = ru;
+          l.rl[i] = (T)(rl > -kInf ? rl * rf : -kInf);
+          l.ru[i] = (T)(ru < kInf ? ru * rf : kInf);
+          if (rl > -kInf) part[0] = nanmax(part[0], fabs(rl));
+          if (ru < kInf) part[0] = nanmax(part[0], fabs(ru));
+          const double b = (rl > -kInf ? (double)l.rl[i] : 0.0) + (ru < kInf ? (double)l.ru[i] : 0.0);
+          part[1] += b * b;     // ||b'||^2 as the scaling phase forms it
+          l.yr[i] = l.y[i];     // the dual carries over (warm start)
+          l.sumy[i] = T(0);
+        }
+      } break;
+
       default: break;
     }
   }
@@ -496,7 +636,7 @@ struct PhaseWork {
     constexpr bool kDerived = sizeof(T) == 8;
 
     auto unscaled_row = [&](int i, double au, double aua, double y, double ya) {
-      const double rf = l.rowf[i], rl0 = l.rl0[i], ru0 = l.ru0[i];
+      const double rf = l.rowf[i], rl0 = l.crl[i], ru0 = l.cru[i];
       const double yu = (y * rf) / so, yua = (ya * rf) / so;
       v[PUC] = nanmax(v[PUC], fabs(fmax(rl0 - au, 0.0) + fmax(au - ru0, 0.0)));
       v[SUUC] += bsel_row(yu, rl0, ru0) * yu;
@@ -504,7 +644,7 @@ struct PhaseWork {
       v[SUUA] += bsel_row(yua, rl0, ru0) * yua;
     };
     auto unscaled_col = [&](int j, double atu, double atua, double x, double xa) {
-      const double cf = l.colf[j], lo0 = l.lo0[j], hi0 = l.hi0[j], c0 = l.c0[j];
+      const double cf = l.colf[j], lo0 = l.clo[j], hi0 = l.chi[j], c0 = l.c0[j];
       const double xu = x * cf, xua = xa * cf;
       const double du = c0 + atu, dua = c0 + atua;
       v[PUC] = nanmax(v[PUC], fabs(fmax(lo0 - xu, 0.0) + fmax(xu - hi0, 0.0)));
@@ -606,14 +746,17 @@ struct Transition {
     return sqrt(a * a + b * b + g * g);
   }
 
-  __device__ __forceinline__ int next_mode(long long k1) const {
-    if (k1 % P.check_every == 0 || k1 >= P.max_iter) return NEXT_NONE;
+  __device__ __forceinline__ bool refining() const { return P.refine_rounds > 0; }
+
+  __device__ __forceinline__ int next_mode(const LpState& s, long long k1) const {
+    if (k1 % P.check_every == 0 || k1 >= s.kmax) return NEXT_NONE;
     return (k1 % P.stepsize_every == 0) ? NEXT_TRIAL : NEXT_FUSED;
   }
 
   // top of the loop body (pdlp.py:313-317, 335): limits, then the step flavour
+  // (the iteration limit counts the iterations of every refinement round)
   __device__ __forceinline__ int loop_top(LpState& s) const {
-    if (s.k >= P.max_iter) {
+    if (s.k >= s.kmax) {
       s.status = ST_ITERATION_LIMIT;
       return PH_FINAL;
     }
@@ -631,7 +774,7 @@ struct Transition {
       return PH_P_TRIAL;
     }
     s.primal_ready = 0;
-    s.next_mode = next_mode(s.k + 1);
+    s.next_mode = next_mode(s, s.k + 1);
     return PH_A_FUSED;
   }
 
@@ -649,7 +792,7 @@ struct Transition {
   // Called by one thread with `s` and `t` in LDS (out of line: the scalar logic
   // then costs the phase kernels neither registers nor scratch).
   __device__ __noinline__ int apply(LpState& s, int ph, const double* t, double* hist,
-                                    long long hist_cap) const {
+                                    long long hist_cap, double* rhist) const {
     switch (ph) {
       case PH_INIT: s.sweep = 0; return PH_RUIZ_MAX;
       case PH_RUIZ_MAX: s.dev = t[0]; return PH_RUIZ_APPLY;
@@ -667,15 +810,32 @@ struct Transition {
         return PH_SCALE;
       }
       case PH_SCALE: {
-        s.ptol = fmin(P.eps_abs + P.eps_rel * t[4], P.feas_cap);
-        s.dtol = fmin(P.eps_abs + P.eps_rel * t[3], P.feas_cap);
+        s.ptol0 = fmin(P.eps_abs + P.eps_rel * t[4], P.feas_cap);
+        s.dtol0 = fmin(P.eps_abs + P.eps_rel * t[3], P.feas_cap);
+        s.rhs0 = t[4];
+        s.ref_scale = 1.0;
+        s.round = 0;
+        s.absorbed = 0;
+        s.k_done = 0;
+        s.kmax = P.max_iter;
+        if (refining()) {
+          // a round is solved to a relative tolerance that FP32 iterates can
+          // reach; the absolute feasibility cap is met by the rounds together
+          s.ptol = P.refine_eps + P.refine_eps * t[4];
+          s.dtol = P.refine_eps + P.refine_eps * t[3];
+        } else {
+          s.ptol = s.ptol0;
+          s.dtol = s.dtol0;
+        }
         const double cn = sqrt(t[0]), bn = sqrt(t[1]);
+        s.cnorm = cn;
         s.nv = sqrt(t[2]);
         const double w = bn > 0.0 ? exp(log(fmax(cn, 1e-12)) - log(fmax(bn, 1e-12))) : 1.0;
         s.omega = fmin(fmax(w, 1e-4), 1e4);
         return PH_AX0;
       }
       case PH_AX0:
+        if (s.round > 0) return PH_KKT0;   // refinement round: ||A|| and eta carry over
         if (s.nv != 0.0) {
           s.pit = 0;
           s.est = 1.0;
@@ -704,7 +864,8 @@ struct Transition {
         s.mu_restart = merit(s, t[0], t[1], t[2], t[3], t[4], s.omega);
         s.mu_last = __builtin_inf();
         s.sum_eta = 0.0;
-        s.k = s.epoch = s.attempts = s.restarts = s.rejected = s.hist_len = 0;
+        s.k = s.epoch = s.attempts = 0;
+        if (s.round == 0) s.restarts = s.rejected = s.hist_len = 0;
         s.status = ST_RUNNING;
         s.time_flag = t[5] > 0.0;
         s.primal_ready = 0;
@@ -726,7 +887,7 @@ struct Transition {
           s.eta = nxt;
         }
         if (accepted) {
-          s.next_mode = next_mode(s.k + 1);
+          s.next_mode = next_mode(s, s.k + 1);
           return PH_B_ADAPT;
         }
         ++s.trial;
@@ -754,14 +915,17 @@ struct Transition {
         const double gu = fabs(pou - dou);
         s.use_avg = use_avg;
         s.primal_ready = 0;
-        if (pu <= s.ptol && du <= s.dtol && gu <= P.eps_abs + P.eps_rel * (fabs(pou) + fabs(dou))) {
+        const double ea = refining() ? P.refine_eps : P.eps_abs;
+        const double er = refining() ? P.refine_eps : P.eps_rel;
+        if (pu <= s.ptol && du <= s.dtol && gu <= ea + er * (fabs(pou) + fabs(dou))) {
+          if (refining()) return use_avg ? PH_COPY_AVG : PH_REF_X;   // round converged
           s.status = ST_OPTIMAL;
           s.fin[0] = pu; s.fin[1] = du; s.fin[2] = gu; s.fin[3] = pou; s.fin[4] = dou;
           return use_avg ? PH_COPY_AVG : PH_FINAL;
         }
         if (s.hist_len < hist_cap) {
           double* h = hist + 5 * s.hist_len;
-          h[0] = (double)s.k; h[1] = pu; h[2] = du; h[3] = gu; h[4] = pou;
+          h[0] = (double)(s.k_done + s.k); h[1] = pu; h[2] = du; h[3] = gu; h[4] = pou;
         }
         ++s.hist_len;
         const bool restart = mu_cand <= P.beta_sufficient * s.mu_restart ||
@@ -785,7 +949,66 @@ struct Transition {
         ++s.restarts;
         return loop_top(s);
       }
-      case PH_COPY_AVG: return PH_FINAL;
+      case PH_COPY_AVG: return s.status == ST_RUNNING && refining() ? PH_REF_X : PH_FINAL;
+      case PH_REF_X:
+        s.absorbed = 1;
+        return PH_REF_KKT;
+      case PH_REF_KKT: {
+        // X against the original problem: the reference's _converged test
+        // (pdlp.py:523-530) plus the refinement target on the primal residual
+        s.time_flag = t[5] > 0.0;
+        const double pres = t[0], dres = t[1];
+        const double pou = t[2] + offset0, dou = (-t[3] + t[4]) + offset0;
+        const double gap = fabs(pou - dou);
+        if (s.round < kMaxRounds) {
+          double* h = rhist + 5 * s.round;
+          h[0] = (double)(s.k_done + s.k); h[1] = pres; h[2] = dres; h[3] = gap; h[4] = pou;
+          s.ref_rows = s.round + 1;
+        }
+        s.k_done += s.k;
+        s.k = 0;
+        const bool conv = pres <= s.ptol0 && dres <= s.dtol0 &&
+                          gap <= P.eps_abs + P.eps_rel * (fabs(pou) + fabs(dou));
+        const bool target = pres <= P.refine_tol * (1.0 + s.rhs0);
+        if (conv && (target || s.round + 1 >= P.refine_rounds)) {
+          s.status = ST_OPTIMAL;
+          s.fin[0] = pres; s.fin[1] = dres; s.fin[2] = gap; s.fin[3] = pou; s.fin[4] = dou;
+          return PH_FINAL;
+        }
+        if (s.k_done >= P.max_iter) {
+          s.status = ST_ITERATION_LIMIT;
+          return PH_FINAL;
+        }
+        if (s.time_flag) {
+          s.status = ST_TIME_LIMIT;
+          return PH_FINAL;
+        }
+        if (s.round + 1 >= P.refine_rounds) {
+          s.status = ST_NUMERICAL;   // the rounds could not reach the tolerance
+          return PH_FINAL;
+        }
+        // Delta: the power of two nearest below 1 / residual, at most 2^20
+        // times the last one (and never below 1)
+        double d = exp2(floor(log2(1.0 / fmax(pres, 1e-300))));
+        d = fmin(d, s.ref_scale * 1048576.0);
+        s.ref_scale = fmax(d, 1.0);
+        ++s.round;
+        return PH_REF_SHIFT;
+      }
+      case PH_REF_SHIFT: {
+        // tolerances of the new round: relative to its own right-hand side,
+        // capped at the original problem's scale
+        s.ptol = fmin(P.refine_eps + P.refine_eps * t[0], P.refine_eps * (1.0 + s.rhs0));
+        // step size and primal weight start as for a new problem (pdlp.py:
+        // 289-296): the primal is rescaled by Delta, the dual is not
+        const double bn = sqrt(t[1]);
+        const double w = bn > 0.0 ? exp(log(fmax(s.cnorm, 1e-12)) - log(fmax(bn, 1e-12))) : 1.0;
+        s.omega = fmin(fmax(w, 1e-4), 1e4);
+        s.eta = 1.0 / s.anorm;
+        s.absorbed = 0;
+        s.kmax = P.max_iter - s.k_done;
+        return PH_AX0;
+      }
       case PH_FINAL: {
         // objective of the caller's problem (pdlp.py:473): prob.c = -c0 when maximising
         const double cx = flip ? -t[0] : t[0];

@@ -124,19 +124,34 @@ __device__ __forceinline__ void block_combine(T (&s)[NV], T* scratch) {
 //   idx   : column index of each nonzero
 //   tiles : tile t covers rows [tiles[t], tiles[t+1])
 //   prod(q, j, v[NV]) writes the NV products of nonzero q (column j)
-//   epi(row, s[NV]) receives the NV row reductions (sum, or max if MAXOP)
+//   pre(row) -> R     loads what the row's epilogue needs; it is called
+//                     before the products are staged, so those loads are in
+//                     flight together with the product loads
+//   epi(row, s[NV], r) receives the NV row reductions (sum, or max if MAXOP)
+//                     and pre's result
+// The bounds of tile t+1 are fetched while tile t is processed.  A workgroup
+// runs the tiles one after another, so the number of dependent memory
+// latencies per tile, not bandwidth, sets the time of a small pass.
 // ------------------------------------------------------------------------
-template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Prod, class Epi>
-__device__ __forceinline__ void tile_pass(const int32_t* __restrict__ ptr,
-                                          const int32_t* __restrict__ idx,
-                                          const int32_t* __restrict__ tiles, int t0, int t1,
-                                          Prod& prod, Epi& epi, T* lds) {
-  for (int t = uniform(t0); t < uniform(t1); ++t) {
-    const int r0 = uniform(tiles[t]), r1 = uniform(tiles[t + 1]);
-    const int p0 = uniform(ptr[r0]), p1 = uniform(ptr[r1]);
+template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Prod, class Pre, class Epi>
+__device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
+                                             const int32_t* __restrict__ idx,
+                                             const int32_t* __restrict__ tiles, int t0, int t1,
+                                             Prod& prod, Pre& pre, Epi& epi, T* lds) {
+  t0 = uniform(t0);
+  t1 = uniform(t1);
+  if (t0 >= t1) return;
+  int r0 = uniform(tiles[t0]), r1 = uniform(tiles[t0 + 1]);
+  int p0 = uniform(ptr[r0]), p1 = uniform(ptr[r1]);
+  for (int t = t0; t < t1; ++t) {
+    const bool more = t + 1 < t1;
+    const int nr1 = more ? uniform(tiles[t + 2]) : r1;   // next tile's end row
     const int nz = p1 - p0, rows = r1 - r0;
+    int np1 = p1;
     if (nz > TNNZ) {
       // one long row: the whole workgroup strides over it
+      decltype(pre(0)) pr{};
+      if (threadIdx.x == 0) pr = pre(r0);
       T s[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k) s[k] = T(0);
@@ -146,10 +161,23 @@ __device__ __forceinline__ void tile_pass(const int32_t* __restrict__ ptr,
 #pragma unroll
         for (int k = 0; k < NV; ++k) s[k] = MAXOP ? tmax(s[k], v[k]) : s[k] + v[k];
       }
+      if (more) np1 = uniform(ptr[nr1]);
       block_combine<BLOCK, NV, MAXOP, T>(s, lds);
-      if (threadIdx.x == 0) epi(r0, s);
+      if (threadIdx.x == 0) epi(r0, s, pr);
       __syncthreads();
     } else {
+      int tpr = BLOCK / (rows > 0 ? rows : 1);
+      tpr = tpr >= 64 ? 64 : (tpr >= 32 ? 32 : (tpr >= 16 ? 16 : (tpr >= 8 ? 8 : (tpr >= 4 ? 4 : (tpr >= 2 ? 2 : 1)))));
+      const int grp = (int)threadIdx.x / tpr, ln = (int)threadIdx.x & (tpr - 1);
+      const bool active = grp < rows;
+      // row extent and epilogue operands: issued before the product loads
+      int a = 0, b = 0;
+      decltype(pre(0)) pr{};
+      if (active) {
+        a = ptr[r0 + grp] - p0;
+        b = ptr[r0 + grp + 1] - p0;
+        if (ln == 0) pr = pre(r0 + grp);
+      }
       for (int q = (int)threadIdx.x; q < nz; q += BLOCK) {
         T v[NV];
         prod(p0 + q, idx[p0 + q], v);
@@ -157,15 +185,11 @@ __device__ __forceinline__ void tile_pass(const int32_t* __restrict__ ptr,
         for (int k = 0; k < NV; ++k) lds[k * TNNZ + q] = v[k];
       }
       __syncthreads();
-      int tpr = BLOCK / (rows > 0 ? rows : 1);
-      tpr = tpr >= 64 ? 64 : (tpr >= 32 ? 32 : (tpr >= 16 ? 16 : (tpr >= 8 ? 8 : (tpr >= 4 ? 4 : (tpr >= 2 ? 2 : 1)))));
-      const int grp = (int)threadIdx.x / tpr, ln = (int)threadIdx.x & (tpr - 1);
+      if (more) np1 = uniform(ptr[nr1]);
       T s[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k) s[k] = T(0);
-      const bool active = grp < rows;
       if (active) {
-        const int a = ptr[r0 + grp] - p0, b = ptr[r0 + grp + 1] - p0;
         for (int q = a + ln; q < b; q += tpr) {
 #pragma unroll
           for (int k = 0; k < NV; ++k) s[k] = MAXOP ? tmax(s[k], lds[k * TNNZ + q]) : s[k] + lds[k * TNNZ + q];
@@ -178,10 +202,25 @@ __device__ __forceinline__ void tile_pass(const int32_t* __restrict__ ptr,
           s[k] = MAXOP ? tmax(s[k], u) : s[k] + u;
         }
       }
-      if (active && ln == 0) epi(r0 + grp, s);
+      if (active && ln == 0) epi(r0 + grp, s, pr);
       __syncthreads();
     }
+    r0 = r1;
+    p0 = p1;
+    r1 = nr1;
+    p1 = np1;
   }
+}
+
+// The tile pass without epilogue prefetch: epi(row, s[NV]).
+template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Prod, class Epi>
+__device__ __forceinline__ void tile_pass(const int32_t* __restrict__ ptr,
+                                          const int32_t* __restrict__ idx,
+                                          const int32_t* __restrict__ tiles, int t0, int t1,
+                                          Prod& prod, Epi& epi, T* lds) {
+  auto pre = [](int) { return 0; };
+  auto epi3 = [&](int i, const T (&s)[NV], int) { epi(i, s); };
+  tile_pass_pf<BLOCK, TNNZ, NV, MAXOP, T>(ptr, idx, tiles, t0, t1, prod, pre, epi3, lds);
 }
 
 // Visit every nonzero of a tile range with its row: fn(q, row, col).

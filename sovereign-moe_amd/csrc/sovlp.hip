@@ -5,7 +5,7 @@
 //   elementwise kernels  Backend surface steps (backend.py:204-291)
 //   reduce_kernel        Backend surface dot / nrm2 / nrm_inf, fixed order
 //   spmv_tiled_kernel    the engines' CSR-Adaptive tile pass as a bare SpMV
-//   pdlp_step_kernel     one PDLP phase of each LP, one workgroup per LP
+//   pdlp_run_kernel      PDLP phases of each LP, one workgroup per LP
 //   pdlp_phase_kernel    one PDLP phase of one LP over W workgroups
 //   pdlp_decide_kernel   reduces the W partial sums and takes the transition
 #include <hip/hip_runtime.h>
@@ -208,7 +208,16 @@ void carve_lp(Carver& cv, LpDev<T>& L, int m, int n, long long nnz, long long hi
   L.dout = cv.take<double>(n);
   L.hist = cv.take<double>(5 * (size_t)(hist_cap > 0 ? hist_cap : 1));
   L.hist_cap = hist_cap;
+  L.rhist = cv.take<double>(5 * kMaxRounds);
   L.st = cv.take<LpState>(1);
+  if (sizeof(T) == 4) {      // FP32 iterates: room for iterative refinement
+    L.wrl = cv.take<double>(m);
+    L.wru = cv.take<double>(m);
+    L.wlo = cv.take<double>(n);
+    L.whi = cv.take<double>(n);
+    L.xacc = cv.take<double>(n);
+    L.axacc = cv.take<double>(m);
+  }
 }
 
 int clock_hz(double* hz) {
@@ -452,7 +461,9 @@ struct sovlp_pdlp {
   LpState* st;
   StepCtl* ctl;
   unsigned* done;
-  double *xo, *yo, *dout, *hist;
+  double *xo, *yo, *dout, *hist, *rhist;
+  long long* pcount;         // [kPhaseSlots]
+  int phases_per_replay;
   long long hist_cap;
   double hz;
   bool started;
@@ -460,6 +471,8 @@ struct sovlp_pdlp {
   hipGraphExec_t exec;
   hipStream_t cap;
   long long ticks_set;
+  LpState* hstate;           // [2] pinned host copies of the state
+  hipEvent_t ev[2];
 };
 
 struct sovlp_batch {
@@ -467,19 +480,23 @@ struct sovlp_batch {
   void* lps;                 // LpDev<T>[count] (device)
   StepCtl* ctl;
   unsigned* done;
-  std::vector<double*> xo, yo, dout;
+  long long* pcount;         // [count][kPhaseSlots]
+  std::vector<double*> xo, yo, dout, rhist;
   std::vector<LpState*> st;
   hipGraph_t graph;
   hipGraphExec_t exec;
   hipStream_t cap;
   double hz;
   long long ticks_set;
+  unsigned* hdone;           // [2] pinned host copies of the done counter
+  hipEvent_t ev[2];
 };
 
 namespace {
 
 constexpr int kTnnzSmall = 512;   // 64-thread workgroups: 2 x 512 x 8 B = 8 KiB LDS
-constexpr int kStepsPerGraph = 64;
+constexpr int kPairsPerGraph = 64;     // W > 1: phase + decide launches per graph
+constexpr int kPhasesPerLaunch = 128;  // run kernel: phases per LP per launch
 
 template <class T>
 void fill_lp(LpDev<T>& L, const sovlp_lp& lp) {
@@ -491,6 +508,11 @@ void fill_lp(LpDev<T>& L, const sovlp_lp& lp) {
   L.c0 = lp.c; L.lo0 = lp.lo; L.hi0 = lp.hi; L.rl0 = lp.rl; L.ru0 = lp.ru; L.v0 = lp.v0;
   L.tilesA = lp.tiles_a;
   L.tilesT = lp.tiles_t;
+  if (L.xacc) {              // the rounds' bounds live in the workspace
+    L.crl = L.wrl; L.cru = L.wru; L.clo = L.wlo; L.chi = L.whi;
+  } else {                   // no refinement: the original bounds
+    L.crl = lp.rl; L.cru = lp.ru; L.clo = lp.lo; L.chi = lp.hi;
+  }
 }
 
 // workgroups sharing one LP's phases: one per ~4 tiles, at most what is resident
@@ -519,6 +541,7 @@ size_t single_ws_T(const sovlp_lp* lp, long long hist_cap, int W) {
   cv.take<double>((size_t)W * kMaxRed);
   cv.take<int32_t>(W + 1);
   cv.take<int32_t>(W + 1);
+  cv.take<long long>(kPhaseSlots);
   LpDev<T> L{};
   carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
   return cv.off + 256;
@@ -538,10 +561,12 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   double* partials = cv.take<double>((size_t)W * kMaxRed);
   int32_t* partA = cv.take<int32_t>(W + 1);
   int32_t* partT = cv.take<int32_t>(W + 1);
+  long long* pcount = cv.take<long long>(kPhaseSlots);
   LpDev<T> L{};
   carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
   fill_lp<T>(L, *lp);
   L.partials = partials;
+  L.pcount = pcount;
   if (W > 1) {
     // equal-cost split of both tile lists (needs the pointer values at tile starts)
     std::vector<int32_t> ta(lp->ntiles_a + 1), tt(lp->ntiles_t + 1), rp(lp->m + 1), cp(lp->n + 1);
@@ -571,6 +596,7 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   HIPCHK(hipMemcpyAsync(L.st, &s0, sizeof s0, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(done, 0, sizeof(unsigned), st));
+  HIPCHK(hipMemsetAsync(pcount, 0, sizeof(long long) * kPhaseSlots, st));
   HIPCHK(hipMemcpyAsync(d_lp, &L, sizeof L, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
 
@@ -582,23 +608,30 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   h->st = L.st;
   h->ctl = ctl;
   h->done = done;
-  h->xo = L.xo; h->yo = L.yo; h->dout = L.dout; h->hist = L.hist;
+  h->xo = L.xo; h->yo = L.yo; h->dout = L.dout; h->hist = L.hist; h->rhist = L.rhist;
+  h->pcount = pcount;
   h->hist_cap = hist_cap;
   h->ticks_set = c0.ticks;
-  // record kStepsPerGraph phases into a graph on a private capture stream
+  // record the launches of one replay into a graph on a private capture
+  // stream: W == 1, one run launch (kPhasesPerLaunch phases); W > 1,
+  // kPairsPerGraph phase + decide pairs (one phase each)
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
   constexpr int TN = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
-  for (int i = 0; i < kStepsPerGraph; ++i) {
-    if (W == 1) {
-      pdlp_step_kernel<T, kBlock, TN><<<1, kBlock, 0, h->cap>>>(d_lp, ctl, done);
-    } else {
+  if (W == 1) {
+    pdlp_run_kernel<T, kBlock, TN><<<1, kBlock, 0, h->cap>>>(d_lp, ctl, done, kPhasesPerLaunch);
+    h->phases_per_replay = kPhasesPerLaunch;
+  } else {
+    for (int i = 0; i < kPairsPerGraph; ++i) {
       pdlp_phase_kernel<T, kBlock, TN><<<W, kBlock, 0, h->cap>>>(d_lp);
       pdlp_decide_kernel<T, kBlock><<<1, kBlock, 0, h->cap>>>(d_lp, W, ctl, done);
     }
+    h->phases_per_replay = kPairsPerGraph;
   }
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  HIPCHK(hipHostMalloc((void**)&h->hstate, 2 * sizeof(LpState), hipHostMallocDefault));
+  for (hipEvent_t& e : h->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return clock_hz(&h->hz);
 }
 
@@ -608,6 +641,7 @@ size_t batch_ws_T(int count, const sovlp_lp* lps, long long hist_cap) {
   cv.take<LpDev<T>>(count);
   cv.take<StepCtl>(1);
   cv.take<unsigned>(64);
+  cv.take<long long>((size_t)count * kPhaseSlots);
   for (int i = 0; i < count; ++i) {
     LpDev<T> L{};
     carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
@@ -623,10 +657,12 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, int 
   LpDev<T>* d_lps = cv.take<LpDev<T>>(count);
   StepCtl* ctl = cv.take<StepCtl>(1);
   unsigned* done = cv.take<unsigned>(64);
+  long long* pcount = cv.take<long long>((size_t)count * kPhaseSlots);
   std::vector<LpDev<T>> host(count);
   h->xo.resize(count);
   h->yo.resize(count);
   h->dout.resize(count);
+  h->rhist.resize(count);
   h->st.resize(count);
   for (int i = 0; i < count; ++i) {
     if (lps[i].nnz >= (int64_t)INT32_MAX) return fail(4, "an LP has more than 2^31-1 nonzeros");
@@ -634,7 +670,9 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, int 
     L = LpDev<T>{};
     carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
     fill_lp<T>(L, lps[i]);
+    L.pcount = pcount + (size_t)i * kPhaseSlots;
     h->xo[i] = L.xo; h->yo[i] = L.yo; h->dout[i] = L.dout; h->st[i] = L.st;
+    h->rhist[i] = L.rhist;
   }
   StepCtl c0{};
   std::memcpy(&c0.P, prm, sizeof(Params));
@@ -648,18 +686,20 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, int 
   h->lps = d_lps;
   h->ctl = ctl;
   h->done = done;
+  h->pcount = pcount;
   h->ticks_set = c0.ticks;
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < kStepsPerGraph; ++i) {
-    if (block == 64)
-      pdlp_step_kernel<T, 64, kTnnzSmall><<<count, 64, 0, h->cap>>>(d_lps, ctl, done);
-    else
-      pdlp_step_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>
-          <<<count, kBlock, 0, h->cap>>>(d_lps, ctl, done);
-  }
+  if (block == 64)
+    pdlp_run_kernel<T, 64, kTnnzSmall><<<count, 64, 0, h->cap>>>(d_lps, ctl, done,
+                                                                  kPhasesPerLaunch);
+  else
+    pdlp_run_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>
+        <<<count, kBlock, 0, h->cap>>>(d_lps, ctl, done, kPhasesPerLaunch);
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  HIPCHK(hipHostMalloc((void**)&h->hdone, 2 * sizeof(unsigned), hipHostMallocDefault));
+  for (hipEvent_t& e : h->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return clock_hz(&h->hz);
 }
 
@@ -679,7 +719,9 @@ void fill_info(const LpState& t, sovlp_info* f) {
   const bool opt = t.status == ST_OPTIMAL;
   const double nan = std::nan("");
   f->status = t.status;
-  f->stage = t.phase == PH_DONE ? 3 : (t.phase < PH_KKT0 ? 0 : (t.phase >= PH_FINAL ? 2 : 1));
+  f->stage = t.phase == PH_DONE ? 3
+           : t.phase < PH_KKT0 ? 0
+           : (t.phase == PH_FINAL || t.phase == PH_FINAL_D) ? 2 : 1;
   f->iterations = t.k;
   f->restarts = t.restarts;
   f->rejected_steps = t.rejected;
@@ -694,6 +736,21 @@ void fill_info(const LpState& t, sovlp_info* f) {
   f->objective = t.objective;
   f->scale_obj = t.sobj;
   f->device_ticks = t.ticks;
+  f->iterations = t.k_done + t.k;
+  f->refine_rows = t.ref_rows;
+  f->refine_round = t.round;
+  f->refine_scale = t.ref_scale;
+}
+
+int check_params(const sovlp_params* p, int precision) {
+  if (p->check_every < 1 || p->stepsize_every < 1) return fail(1, "check/stepsize period < 1");
+  if (p->refine_rounds < 0 || p->refine_rounds > kMaxRounds)
+    return fail(1, "refine_rounds must be in [0, 16]");
+  if (p->refine_rounds > 0 && precision != 32)
+    return fail(4, "iterative refinement is provided for precision 32 only");
+  if (p->refine_rounds > 0 && !(p->refine_eps > 0.0 && p->refine_tol >= 0.0))
+    return fail(1, "refine_eps must be > 0 and refine_tol >= 0");
+  return 0;
 }
 
 }  // namespace
@@ -719,8 +776,8 @@ int sovlp_pdlp_create(const sovlp_lp* lp, const sovlp_params* params, int precis
   if (!lp || !params || !workspace || !out) return fail(1, "null argument");
   if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) return fail(1, "negative dimension");
   if (lp->nnz >= (int64_t)INT32_MAX) return fail(4, "more than 2^31-1 nonzeros");
-  if (params->check_every < 1 || params->stepsize_every < 1) return fail(1, "check/stepsize period < 1");
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  if (int rc = check_params(params, precision)) return rc;
   if (team < 0 || team > 2) return fail(1, "team must be 0 (auto), 1 (one workgroup) or 2 (many)");
   sovlp_pdlp* h = new sovlp_pdlp{};
   int rc = precision == 64
@@ -745,21 +802,38 @@ int sovlp_pdlp_advance(sovlp_pdlp* h, int64_t iteration_budget, double seconds_l
     if (rc) return rc;
     h->started = true;
   }
-  LpState s;
-  HIPCHK(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&h->hstate[0], h->st, sizeof(LpState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  const long long k0 = s.k;
+  const long long k0 = h->hstate[0].k;
   const long long budget = iteration_budget < 1 ? 1 : iteration_budget;
-  // every replay runs kStepsPerGraph phases; an iteration takes at most 61
+  // every replay runs phases_per_replay phases; an iteration takes at most 61
   // phases (60 rejected trials), so this bound is never the one that stops a
   // healthy solve
-  const long long max_replays = (budget * 61 + 200) / kStepsPerGraph + 16;
+  const long long max_replays = (budget * 61 + 200) / h->phases_per_replay + 16;
+  // one replay stays queued while the host reads the state the previous one
+  // left (pinned copy + event), so the device does not idle on the host
   long long replays = 0;
-  while (s.phase != PH_DONE && s.k - k0 < budget && replays++ < max_replays) {
-    HIPCHK(hipGraphLaunch(h->exec, st));
-    HIPCHK(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+  int cur = 0;
+  if (h->hstate[0].phase == PH_DONE) return 0;
+  HIPCHK(hipGraphLaunch(h->exec, st));
+  HIPCHK(hipMemcpyAsync(&h->hstate[0], h->st, sizeof(LpState), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(h->ev[0], st));
+  ++replays;
+  for (;;) {
+    const int nxt = cur ^ 1;
+    const bool more = replays < max_replays;
+    if (more) {
+      HIPCHK(hipGraphLaunch(h->exec, st));
+      HIPCHK(hipMemcpyAsync(&h->hstate[nxt], h->st, sizeof(LpState), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipEventRecord(h->ev[nxt], st));
+      ++replays;
+    }
+    HIPCHK(hipEventSynchronize(h->ev[cur]));
+    const LpState& s = h->hstate[cur];
+    if (s.phase == PH_DONE || s.k - k0 >= budget || !more) break;
+    cur = nxt;
   }
+  HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
 
@@ -781,14 +855,47 @@ int sovlp_pdlp_outputs(sovlp_pdlp* h, double** x, double** y, double** d, double
   return 0;
 }
 
+int sovlp_pdlp_refinement(sovlp_pdlp* h, double** rows) {
+  if (!h || !rows) return fail(1, "null argument");
+  *rows = h->rhist;
+  return 0;
+}
+
 int sovlp_pdlp_width(sovlp_pdlp* h, int* workgroups) {
   if (!h || !workgroups) return fail(1, "null argument");
   *workgroups = h->W;
   return 0;
 }
 
+int sovlp_phase_slots(int* slots) {
+  if (!slots) return fail(1, "null argument");
+  *slots = kPhaseSlots;
+  return 0;
+}
+
+const char* sovlp_phase_name(int slot) {
+  static const char* names[kPhaseSlots] = {
+      "INIT", "RUIZ_MAX", "RUIZ_APPLY", "PC_SUM", "PC_APPLY", "SCALE", "AX0", "POW_A",
+      "POW_T", "KKT0", "P_TRIAL", "A_TRIAL", "B_ADAPT", "A_FUSED", "B_FUSED", "CHECK",
+      "RESTART", "COPY_AVG", "FINAL", "FINAL_D", "REF_X", "REF_KKT", "REF_SHIFT", "DONE",
+      "B_ADAPT_NEXT", "B_FUSED_NEXT", "", ""};
+  return slot >= 0 && slot < kPhaseSlots ? names[slot] : "";
+}
+
+int sovlp_pdlp_phase_counts(sovlp_pdlp* h, int64_t* counts, sovlp_stream_t stream) {
+  if (!h || !counts) return fail(1, "null argument");
+  static_assert(sizeof(long long) == sizeof(int64_t), "counter width");
+  HIPCHK(hipMemcpyAsync(counts, h->pcount, sizeof(long long) * kPhaseSlots,
+                        hipMemcpyDeviceToHost, S(stream)));
+  HIPCHK(hipStreamSynchronize(S(stream)));
+  return 0;
+}
+
 int sovlp_pdlp_destroy(sovlp_pdlp* h) {
   if (!h) return 0;
+  for (hipEvent_t e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->hstate) (void)hipHostFree(h->hstate);
   if (h->exec) (void)hipGraphExecDestroy(h->exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->cap) (void)hipStreamDestroy(h->cap);
@@ -819,7 +926,7 @@ int sovlp_batch_create(int count, const sovlp_lp* lps, const sovlp_params* param
   if (count < 1 || !lps || !params || !workspace || !out) return fail(1, "bad batch arguments");
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
   if (block != 64 && block != 256) return fail(1, "block must be 64 or 256");
-  if (params->check_every < 1 || params->stepsize_every < 1) return fail(1, "check/stepsize period < 1");
+  if (int rc = check_params(params, precision)) return rc;
   sovlp_batch* h = new sovlp_batch{};
   int rc = precision == 64
                ? batch_create_T<double>(count, lps, params, block, workspace, workspace_bytes,
@@ -847,14 +954,28 @@ int sovlp_batch_run(sovlp_batch* h, double seconds_per_lp, int64_t max_graphs,
     state_reset_kernel<float><<<(h->count + 255) / 256, 256, 0, st>>>(
         (const LpDev<float>*)h->lps, h->count, h->done);
   HIPCHK(hipGetLastError());
-  unsigned done = 0;
+  // one replay stays queued while the host reads the done counter the
+  // previous one left (pinned copy + event)
   long long g = 0;
-  while (done < (unsigned)h->count && (max_graphs <= 0 || g < max_graphs)) {
-    HIPCHK(hipGraphLaunch(h->exec, st));
-    ++g;
-    HIPCHK(hipMemcpyAsync(&done, h->done, sizeof done, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+  int cur = 0;
+  HIPCHK(hipGraphLaunch(h->exec, st));
+  HIPCHK(hipMemcpyAsync(&h->hdone[0], h->done, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(h->ev[0], st));
+  ++g;
+  for (;;) {
+    const int nxt = cur ^ 1;
+    const bool more = max_graphs <= 0 || g < max_graphs;
+    if (more) {
+      HIPCHK(hipGraphLaunch(h->exec, st));
+      HIPCHK(hipMemcpyAsync(&h->hdone[nxt], h->done, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipEventRecord(h->ev[nxt], st));
+      ++g;
+    }
+    HIPCHK(hipEventSynchronize(h->ev[cur]));
+    if (h->hdone[cur] >= (unsigned)h->count || !more) break;
+    cur = nxt;
   }
+  HIPCHK(hipStreamSynchronize(st));
   if (graphs_run) *graphs_run = g;
   return 0;
 }
@@ -877,8 +998,25 @@ int sovlp_batch_outputs(sovlp_batch* h, int index, double** x, double** y, doubl
   return 0;
 }
 
+int sovlp_batch_refinement(sovlp_batch* h, int index, double** rows) {
+  if (!h || !rows || index < 0 || index >= h->count) return fail(1, "bad batch index");
+  *rows = h->rhist[index];
+  return 0;
+}
+
+int sovlp_batch_phase_counts(sovlp_batch* h, int64_t* counts, sovlp_stream_t stream) {
+  if (!h || !counts) return fail(1, "null argument");
+  HIPCHK(hipMemcpyAsync(counts, h->pcount, sizeof(long long) * kPhaseSlots * (size_t)h->count,
+                        hipMemcpyDeviceToHost, S(stream)));
+  HIPCHK(hipStreamSynchronize(S(stream)));
+  return 0;
+}
+
 int sovlp_batch_destroy(sovlp_batch* h) {
   if (!h) return 0;
+  for (hipEvent_t e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->hdone) (void)hipHostFree(h->hdone);
   if (h->exec) (void)hipGraphExecDestroy(h->exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->cap) (void)hipStreamDestroy(h->cap);

@@ -29,7 +29,8 @@ class sovlp_params(C.Structure):
     _fields_ = [("eps_abs", F64), ("eps_rel", F64), ("feas_cap", F64), ("max_iter", I64),
                 ("time_limit", F64), ("check_every", I32), ("stepsize_every", I32),
                 ("beta_sufficient", F64), ("beta_necessary", F64), ("beta_artificial", F64),
-                ("primal_weight_smoothing", F64)]
+                ("primal_weight_smoothing", F64), ("refine_rounds", I32), ("refine_pad", I32),
+                ("refine_eps", F64), ("refine_tol", F64)]
 
 
 class sovlp_lp(C.Structure):
@@ -45,7 +46,8 @@ class sovlp_info(C.Structure):
                 ("rejected_steps", I64), ("history_len", I64), ("primal_res", F64),
                 ("dual_res", F64), ("gap", F64), ("primal_obj", F64), ("dual_obj", F64),
                 ("step_size", F64), ("primal_weight", F64), ("objective", F64),
-                ("scale_obj", F64), ("device_ticks", I64)]
+                ("scale_obj", F64), ("device_ticks", I64), ("refine_rows", I32),
+                ("refine_round", I32), ("refine_scale", F64)]
 
 
 # name -> (restype, argtypes); every entry point of include/sovlp.h
@@ -78,7 +80,11 @@ _SIGS = {
     "sovlp_pdlp_info": (C.c_int, [P, C.POINTER(sovlp_info), P]),
     "sovlp_pdlp_outputs": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
     "sovlp_pdlp_width": (C.c_int, [P, C.POINTER(C.c_int)]),
+    "sovlp_pdlp_refinement": (C.c_int, [P, C.POINTER(P)]),
     "sovlp_pdlp_destroy": (C.c_int, [P]),
+    "sovlp_phase_slots": (C.c_int, [C.POINTER(C.c_int)]),
+    "sovlp_phase_name": (C.c_char_p, [C.c_int]),
+    "sovlp_pdlp_phase_counts": (C.c_int, [P, P, P]),
     "sovlp_batch_geometry": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sovlp_batch_workspace_bytes": (C.c_int, [C.c_int, P, C.c_int, I64, C.POINTER(I64)]),
     "sovlp_batch_create": (C.c_int, [C.c_int, P, C.POINTER(sovlp_params), C.c_int, C.c_int, P,
@@ -86,6 +92,8 @@ _SIGS = {
     "sovlp_batch_run": (C.c_int, [P, F64, I64, P, C.POINTER(I64)]),
     "sovlp_batch_infos": (C.c_int, [P, P, P]),
     "sovlp_batch_outputs": (C.c_int, [P, C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+    "sovlp_batch_refinement": (C.c_int, [P, C.c_int, C.POINTER(P)]),
+    "sovlp_batch_phase_counts": (C.c_int, [P, P, P]),
     "sovlp_batch_destroy": (C.c_int, [P]),
 }
 

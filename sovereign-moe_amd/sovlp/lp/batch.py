@@ -27,9 +27,9 @@ import numpy as np
 from .. import _lib
 from ..core.backend import get_backend
 from ..core.problem import ObjSense, Problem, Solution, Status
-from .pdlp import PDLPInfo, PDLPParams, plan_tiles
+from .pdlp import PDLPInfo, PDLPParams, plan_tiles, refinement_rows
 
-__all__ = ["BatchResult", "solve_batch", "DeviceBatch", "batch_geometry"]
+__all__ = ["BatchResult", "BatchEngine", "solve_batch", "DeviceBatch", "batch_geometry"]
 
 
 def batch_geometry(precision: int, block: int) -> tuple[int, int]:
@@ -105,72 +105,95 @@ class BatchResult:
     block: int
 
 
-def solve_batch(problems: list[Problem], params: PDLPParams | None = None, precision: int = 32,
-                block: int = 256, history_cap: int = 1, repeat: int = 1,
-                keep_outputs: bool = True) -> BatchResult:
-    """Solve every LP of ``problems`` on the GPU in one batch.
+class BatchEngine:
+    """A batch laid out on the device with its phase graph recorded; ``run``
+    solves every LP from scratch (the same batch can be solved many times)."""
 
-    ``repeat`` > 1 solves the same batch again from scratch (timing). Returns
-    per-LP ``Solution`` objects (same fields as ``solve_pdlp``'s, method
-    ``"pdlp[gpu,batch]"``) and the batch throughput.
-    """
-    import torch
-    params = params or PDLPParams()
-    get_backend(params.device)
-    db = DeviceBatch(problems, precision, block)
-    nbytes = C.c_int64()
-    _lib.call("sovlp_batch_workspace_bytes", db.count, db.descs, int(precision), int(history_cap),
-              C.byref(nbytes))
-    ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device="cuda")
-    cp = params.to_c()
-    h = C.c_void_p()
-    stream = torch.cuda.current_stream().cuda_stream
-    _lib.call("sovlp_batch_create", db.count, db.descs, C.byref(cp), int(precision), int(block),
-              ws.data_ptr(), nbytes.value, int(history_cap), stream, C.byref(h))
-    try:
-        max_graphs = (int(params.max_iter) * 61 + 400) // 64 + 16
+    def __init__(self, problems: list[Problem], params: PDLPParams | None = None,
+                 precision: int = 32, block: int = 256, history_cap: int = 1):
+        import torch
+        self.params = params or PDLPParams()
+        get_backend(self.params.device)
+        self.precision, self.block = int(precision), int(block)
+        self.db = DeviceBatch(problems, precision, block)
+        self.count = self.db.count
+        nbytes = C.c_int64()
+        _lib.call("sovlp_batch_workspace_bytes", self.count, self.db.descs, self.precision,
+                  int(history_cap), C.byref(nbytes))
+        self.ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device="cuda")
+        self.cparams = self.params.to_c(self.precision)
+        self.stream = torch.cuda.current_stream().cuda_stream
+        h = C.c_void_p()
+        _lib.call("sovlp_batch_create", self.count, self.db.descs, C.byref(self.cparams),
+                  self.precision, self.block, self.ws.data_ptr(), nbytes.value, int(history_cap),
+                  self.stream, C.byref(h))
+        self.h = h
+        self.max_graphs = (int(self.params.max_iter) * 61 + 400) // 64 + 16
+
+    def run(self) -> int:
+        """Solves the whole batch (synchronous); returns the graph replays."""
         g = C.c_int64()
-        best = None
-        for _ in range(max(1, repeat)):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            _lib.call("sovlp_batch_run", h, float(params.time_limit), max_graphs, stream,
-                      C.byref(g))
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            best = dt if best is None else min(best, dt)
-        infos = (_lib.sovlp_info * db.count)()
-        _lib.call("sovlp_batch_infos", h, infos, stream)
+        _lib.call("sovlp_batch_run", self.h, float(self.params.time_limit), self.max_graphs,
+                  self.stream, C.byref(g))
+        return int(g.value)
+
+    def infos(self):
+        infos = (_lib.sovlp_info * self.count)()
+        _lib.call("sovlp_batch_infos", self.h, infos, self.stream)
+        return infos
+
+    def phase_counts(self) -> np.ndarray:
+        slots = C.c_int()
+        _lib.call("sovlp_phase_slots", C.byref(slots))
+        out = np.zeros((self.count, slots.value), dtype=np.int64)
+        _lib.call("sovlp_batch_phase_counts", self.h, out.ctypes.data, self.stream)
+        return out
+
+    def sizes(self) -> list[tuple[int, int, int, int, int]]:
+        """(m, n, nnz, tiles of A, tiles of A^T) of every LP."""
+        return [(m, n, nnz, ta, tt) for (m, n, nnz, ta, tt, _o, _f) in self.db.meta]
+
+    def solutions(self, keep_outputs: bool = True) -> tuple[list, np.ndarray]:
+        """Per-LP ``Solution`` objects and device latencies (seconds)."""
+        import torch
+        infos = self.infos()
         hz = C.c_double()
         _lib.call("sovlp_clock_rate", 0, C.byref(hz))
-        sols = []
         if keep_outputs:
             xs, ys, ds = [], [], []
-            base = ws.data_ptr()
-            for i in range(db.count):
+            ws, base = self.ws, self.ws.data_ptr()
+            for i in range(self.count):
                 x, y, d = C.c_void_p(), C.c_void_p(), C.c_void_p()
-                _lib.call("sovlp_batch_outputs", h, i, C.byref(x), C.byref(y), C.byref(d))
-                m, n = db.meta[i][0], db.meta[i][1]
+                _lib.call("sovlp_batch_outputs", self.h, i, C.byref(x), C.byref(y), C.byref(d))
+                m, n = self.db.meta[i][0], self.db.meta[i][1]
                 xs.append(ws[x.value - base: x.value - base + 8 * n])
                 ys.append(ws[y.value - base: y.value - base + 8 * m])
                 ds.append(ws[d.value - base: d.value - base + 8 * n])
             xall = torch.cat(xs).view(torch.float64).cpu().numpy()
             yall = torch.cat(ys).view(torch.float64).cpu().numpy()
             dall = torch.cat(ds).view(torch.float64).cpu().numpy()
-            ox = oy = 0
-        lat = []
-        for i in range(db.count):
+        ox = oy = 0
+        sols, lat = [], []
+        for i in range(self.count):
             inf = infos[i]
-            m, n = db.meta[i][0], db.meta[i][1]
+            m, n = self.db.meta[i][0], self.db.meta[i][1]
             secs = inf.device_ticks / hz.value if hz.value else float("nan")
             lat.append(secs)
             st = Status(int(inf.status)) if inf.status >= 0 else Status.NOT_SOLVED
+            ref = []
+            if inf.refine_rows > 0:
+                r = C.c_void_p()
+                _lib.call("sovlp_batch_refinement", self.h, i, C.byref(r))
+                off = r.value - self.ws.data_ptr()
+                rows = self.ws[off: off + 40 * int(inf.refine_rows)].view(torch.float64)
+                ref = refinement_rows(rows.cpu().numpy().reshape(-1, 5), bool(self.db.meta[i][6]))
             info = PDLPInfo(iterations=int(inf.iterations), restarts=int(inf.restarts),
                             rejected_steps=int(inf.rejected_steps), primal_res=inf.primal_res,
                             dual_res=inf.dual_res, gap=inf.gap, primal_obj=inf.primal_obj,
                             dual_obj=inf.dual_obj, step_size=inf.step_size,
                             primal_weight=inf.primal_weight, scale_obj=inf.scale_obj,
-                            device_seconds=secs, team="batch", precision=int(precision))
+                            device_seconds=secs, team="batch", precision=self.precision,
+                            refinement=ref)
             sol = Solution(status=st, objective=float(inf.objective),
                            iterations=int(inf.iterations), time=secs, method="pdlp[gpu,batch]")
             if keep_outputs:
@@ -183,9 +206,43 @@ def solve_batch(problems: list[Problem], params: PDLPParams | None = None, preci
             sol.work_units = float(inf.iterations)
             sol.info = info
             sols.append(sol.drop_objective_if_unsolved())
+        return sols, np.array(lat)
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            _lib.lib().sovlp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_batch(problems: list[Problem], params: PDLPParams | None = None, precision: int = 32,
+                block: int = 256, history_cap: int = 1, repeat: int = 1,
+                keep_outputs: bool = True) -> BatchResult:
+    """Solve every LP of ``problems`` on the GPU in one batch.
+
+    ``repeat`` > 1 solves the same batch again from scratch (timing). Returns
+    per-LP ``Solution`` objects (same fields as ``solve_pdlp``'s, method
+    ``"pdlp[gpu,batch]"``) and the batch throughput.
+    """
+    import torch
+    eng = BatchEngine(problems, params, precision, block, history_cap)
+    try:
+        best, graphs = None, 0
+        for _ in range(max(1, repeat)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            graphs = eng.run()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        sols, lat = eng.solutions(keep_outputs)
     finally:
-        _lib.lib().sovlp_batch_destroy(h)
-    lat = np.array(lat)
-    return BatchResult(sols, best, int(g.value), db.count / best,
+        eng.close()
+    return BatchResult(sols, best, graphs, eng.count / best,
                        float(np.percentile(lat, 50)), float(np.percentile(lat, 99)),
                        int(precision), int(block))

@@ -52,13 +52,28 @@ class PDLPParams:
     precision: int = 64
     team: str = "auto"
     chunk_iters: int = 8192
+    # iterative refinement of the FP32 engine (rounds: None = none; each
+    # round to relative KKT max(eps_rel, 1e-6))
+    refine_rounds: int | None = None
+    refine_eps: float | None = None
+    refine_tol: float = 1e-9
 
-    def to_c(self) -> _lib.sovlp_params:
+    def resolved_refine(self, precision: int | None = None) -> tuple[int, float]:
+        prec = int(self.precision if precision is None else precision)
+        rounds = self.refine_rounds
+        if rounds is None:
+            rounds = 0
+        eps = self.refine_eps if self.refine_eps is not None else max(self.eps_rel, 1e-6)
+        return int(rounds), float(eps)
+
+    def to_c(self, precision: int | None = None) -> _lib.sovlp_params:
+        rounds, eps = self.resolved_refine(precision)
         return _lib.sovlp_params(self.eps_abs, self.eps_rel, self.feas_cap, int(self.max_iter),
                                  float(self.time_limit), int(self.check_every),
                                  int(self.stepsize_every), self.beta_sufficient,
                                  self.beta_necessary, self.beta_artificial,
-                                 self.primal_weight_smoothing)
+                                 self.primal_weight_smoothing, rounds, 0, eps,
+                                 float(self.refine_tol))
 
 
 @dataclass
@@ -80,6 +95,9 @@ class PDLPInfo:
     team: str = ""
     workgroups: int = 1
     precision: int = 64
+    # per refinement round: (iterations so far, primal residual, dual
+    # residual, gap, objective) of the accumulated X on the original problem
+    refinement: list = field(default_factory=list)
 
 
 _TEAMS = {"auto": 0, "block": 1, "grid": 2}
@@ -121,6 +139,7 @@ class DeviceLP:
             return torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
 
         c = -prob.c if flip else prob.c
+        self.flip = flip
         self.t = dict(
             rp=up(A.rp, np.int32), ri=up(A.ri, np.int32), rx=up(A.rx, np.float64),
             cp=up(A.cp, np.int32), ci=up(A.ci, np.int32), cx=up(A.cx, np.float64),
@@ -136,6 +155,12 @@ class DeviceLP:
             t["v0"].data_ptr(), t["tiles_a"].data_ptr(), self.tiles_a.size - 1,
             t["tiles_t"].data_ptr(), self.tiles_t.size - 1,
             float(-prob.obj_offset if flip else prob.obj_offset), 1 if flip else 0)
+
+
+def refinement_rows(v: np.ndarray, flip: bool) -> list:
+    """Rows of a device refinement record, objective in the caller's sense."""
+    return [(int(r[0]), float(r[1]), float(r[2]), float(r[3]), float(-r[4] if flip else r[4]))
+            for r in v]
 
 
 def upload_problem(prob: Problem, precision: int = 64) -> DeviceLP:
@@ -159,7 +184,7 @@ class Engine:
         _lib.call("sovlp_pdlp_workspace_bytes", C.byref(dlp.desc), prec, self.hist_cap,
                   C.byref(nbytes))
         self.ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device="cuda")
-        self.cparams = params.to_c()
+        self.cparams = params.to_c(prec)
         h = C.c_void_p()
         if params.team not in _TEAMS:
             raise ValueError(f"team must be one of {sorted(_TEAMS)}")
@@ -195,6 +220,15 @@ class Engine:
         m, n = self.dlp.desc.m, self.dlp.desc.n
         return self._view(x.value, n), self._view(y.value, m), self._view(d.value, n), \
             self._view(h.value, 5 * self.hist_cap)
+
+    def refinement(self, rows: int) -> list:
+        """The refinement record (``rows`` rows, as ``sovlp_pdlp_refinement``)."""
+        if rows <= 0:
+            return []
+        r = C.c_void_p()
+        _lib.call("sovlp_pdlp_refinement", self.h, C.byref(r))
+        v = self._view(r.value, 5 * rows).cpu().numpy().reshape(rows, 5)
+        return refinement_rows(v, self.dlp.flip)
 
     def close(self):
         if getattr(self, "h", None) is not None:
@@ -242,6 +276,7 @@ def solve_pdlp(prob: Problem, params: PDLPParams | None = None) -> Solution:
         d_h = d.cpu().numpy().copy()
         nh = min(int(inf.history_len), eng.hist_cap)
         hist_h = hist[: 5 * nh].cpu().numpy().reshape(nh, 5) if nh else np.zeros((0, 5))
+        ref_rows = eng.refinement(int(inf.refine_rows))
     finally:
         eng.close()
     status = Status(int(inf.status))
@@ -257,7 +292,7 @@ def solve_pdlp(prob: Problem, params: PDLPParams | None = None) -> Solution:
                     scale_obj=inf.scale_obj,
                     device_seconds=inf.device_ticks / hz.value if hz.value else float("nan"),
                     team="grid" if width > 1 else "block", workgroups=width,
-                    precision=int(params.precision))
+                    precision=int(params.precision), refinement=ref_rows)
     sol = Solution(status=status, x=x_h, objective=float(inf.objective), y=y_h,
                    reduced_costs=d_h, iterations=int(inf.iterations),
                    time=time.perf_counter() - t0, method=f"pdlp[{bk.device}]")
