@@ -6,7 +6,9 @@ m ~ log-uniform[20, 800] with 60 % equality rows, n = U[2,4] m, 3-8 nonzeros
 per column, values N(0,1), bounds [0, U(5,50)] per LP.  A step solves the
 whole batch once, from scratch (scaling, power iteration, PDHG to termination
 under the reference's _converged test with eps_abs = eps_rel = feas_cap =
-1e-6), with the inputs resident in HBM.
+1e-6), with the inputs resident in HBM.  Default arithmetic: FP32 iterates
+and matrix, FP64 reductions, and an outer iterative refinement (FP64 X, Y)
+that takes each objective to within ~1e-8 of the FP64 optimum.
 
 Beside the headline number the JSON line carries:
   plain_fp64    the same LPs one at a time through the single-LP engine in
@@ -50,7 +52,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lps", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=1000)
-    ap.add_argument("--precision", type=int, default=64, choices=(64, 32))
+    ap.add_argument("--precision", type=int, default=32, choices=(64, 32),
+                    help="32: FP32 iterates + FP64 reductions + iterative refinement (default)")
     ap.add_argument("--block", type=int, default=256, choices=(256, 64))
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--plain-sample", type=int, default=48,
@@ -162,8 +165,9 @@ def main():
 
     sols, lat = eng.solutions(keep_outputs=True)
     opt = sum(1 for s in sols if int(s.status) == 0)
+    # objective error of every LP reported OPTIMAL, against its closed form
     rel = [abs(s.objective - k.f_star) / max(1.0, abs(k.f_star))
-           for s, k in zip(sols, kos) if s.objective is not None]
+           for s, k in zip(sols, kos) if int(s.status) == 0]
     counts = eng.phase_counts()
     step_bytes = bytes_of_counts(counts, eng.sizes(), a.precision)
     launches = sum(graphs)            # one run-kernel launch per graph replay
@@ -184,7 +188,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == 64 else "f32 iterates, f64 reductions",
+        "dtype": "f64" if a.precision == 64 else "f32 iterates, f64 reductions and refinement",
         "data": "synthetic",
         "config": {"workload": f"{a.lps} uneven LPs with known optima, m log-uniform[20,800] "
                                f"(60% eq), n=U[2,4]m, 3-8 nnz/col, bounds [0,U(5,50)], "
@@ -194,7 +198,10 @@ def main():
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "optimal": opt,
-        "max_rel_obj_err_vs_closed_form": float(max(rel)) if rel else None,
+        "max_rel_obj_err_vs_closed_form": float(max(rel)) if rel else None,   # optimal LPs
+        "rel_obj_err_p99": float(np.percentile(rel, 99)) if rel else None,
+        "mean_iterations": float(np.mean([s.iterations for s in sols])),
+        "mean_refinement_rounds": float(np.mean([len(s.info.refinement) for s in sols])),
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None,

@@ -57,6 +57,12 @@ struct Params {              // same layout as sovlp_params in include/sovlp.h
 };
 
 constexpr int kMaxRounds = 16;   // refinement rounds recorded per LP
+// Trust region of a refinement round, in its zoomed units: every finite
+// shifted bound is clipped to [-kTrustRegion, kTrustRegion], so a round can
+// move X by at most kTrustRegion / Delta (~ 4096 x the last residual).  From
+// an (X, Y) that is only accurate to ~1e-4, an unclipped round starts with a
+// box ~1e5 wide in zoomed units and its first steps overshoot across it.
+constexpr double kTrustRegion = 4096.0;
 
 enum Phase : int {
   PH_INIT = 0, PH_RUIZ_MAX, PH_RUIZ_APPLY, PH_PC_SUM, PH_PC_APPLY, PH_SCALE, PH_AX0, PH_POW_A,
@@ -86,12 +92,16 @@ struct LpState {             // everything the loop carries between phases
   // round solves min c'x^ over bounds shifted by X and scaled by ref_scale
   double ref_scale;          // Delta of the current round (power of two)
   double rhs0;               // ||finite row bounds||_inf of the original problem
-  double cnorm;              // ||c'||_2 of the scaled objective
+  double cinf0;              // ||c||_inf of the original problem
+  double gap_last;           // duality gap of the last X, Y on the original problem
+  double sobj0;              // objective scale of the original problem
   double ptol0, dtol0;       // _converged tolerances of the original problem
   long long k_done;          // iterations of finished rounds
   long long kmax;            // iteration limit of the current round
+  long long k_round0;        // iterations of round 0
   int phase, status, sweep, pit, trial, primal_ready, time_flag, use_avg, next_mode;
   int round;                 // refinement round (0: the plain solve)
+  int conv_last;             // the last X evaluated satisfied _converged
   int absorbed;              // xacc already holds the current round's x
   int ref_rows;              // rows written to rhist
 };
@@ -125,6 +135,11 @@ struct LpDev {
   const double *crl, *cru, *clo, *chi;
   double *wrl, *wru, *wlo, *whi;   // writable copies (refinement only)
   double *xacc, *axacc;      // X and A X (refinement only; null otherwise)
+  // primal-dual refinement (FP32 engines): Y, the reduced cost c + A^T Y,
+  // and the current round's objective and row-slack cost, unscaled FP64;
+  // q is the slack cost in scaled units (zero in round 0)
+  double *yacc, *dacc, *cc, *qc;
+  T* q;
   double* rhist;             // [kMaxRounds][5] per round: k, pres, dres, gap, pobj
   LpState* st;
   long long* pcount;         // [kPhaseSlots] phase counters
@@ -160,7 +175,7 @@ __host__ __device__ constexpr uint32_t phase_maxmask(int ph) {
   return ph == PH_RUIZ_MAX ? 1u : ph == PH_PC_APPLY ? 3u : ph == PH_SCALE ? ((1u << 3) | (1u << 4))
        : ph == PH_KKT0 ? (1u | 2u | 32u)
        : ph == PH_REF_KKT ? (1u | 2u | 32u)
-       : ph == PH_REF_SHIFT ? 1u
+       : ph == PH_REF_SHIFT ? (1u | 8u)
        : ph == PH_CHECK ? ((1u << PC) | (1u << DC) | (1u << PA) | (1u << DA) | (1u << PUC) |
                            (1u << DUC) | (1u << PUA) | (1u << DUA) | (1u << TFLAG))
        : 0u;
@@ -194,6 +209,36 @@ struct PhaseWork {
     } else {
       *a = part[rank];
       *b = part[rank + 1];
+    }
+  }
+
+  // the row-slack cost of a refinement round (scaled; zero without refinement)
+  __device__ __forceinline__ T qs(long long i) const {
+    if constexpr (sizeof(T) == 4) return l.q[i];
+    else return T(0);
+  }
+  // the same, unscaled FP64, and the current round's objective
+  __device__ __forceinline__ double qu(long long i) const {
+    if constexpr (sizeof(T) == 4) return l.qc[i];
+    else return 0.0;
+  }
+  __device__ __forceinline__ double cu(long long j) const {
+    if constexpr (sizeof(T) == 4) return l.cc[j];
+    else return l.c0[j];
+  }
+
+  // dual prox of a row: y' = w - sigma clamp((w - q) / sigma, rl, ru)
+  // (backend.py:219-231 with the refinement round's slack cost q).  FP32: when
+  // the clamp is inactive the result is q itself, not w - sigma ((w - q) /
+  // sigma) with its rounding residue: that residue, multiplied by a shifted
+  // row bound of a refinement round, would swamp the round's duality gap.
+  // FP64 keeps the reference's expression.
+  __device__ __forceinline__ T dual_prox(T w, T q, T sg, T rl, T ru) const {
+    const T t = (w - q) / sg;
+    if constexpr (sizeof(T) == 4) {
+      return t < rl ? w - sg * rl : (t > ru ? w - sg * ru : q);
+    } else {
+      return w - sg * clampT<T>(t, rl, ru);
     }
   }
 
@@ -302,6 +347,7 @@ struct PhaseWork {
             l.wlo[j] = lo0;
             l.whi[j] = hi0;
             l.xacc[j] = 0.0;
+            l.cc[j] = l.c0[j];
           }
           l.c[j] = cj;
           l.lo[j] = lo;
@@ -323,6 +369,9 @@ struct PhaseWork {
           if (l.xacc) {
             l.wrl[i] = rl0;
             l.wru[i] = ru0;
+            l.yacc[i] = 0.0;
+            l.qc[i] = 0.0;
+            l.q[i] = T(0);
           }
           l.rl[i] = rl;
           l.ru[i] = ru;
@@ -366,9 +415,11 @@ struct PhaseWork {
 
       case PH_KKT0: {       // scaled KKT of (x0, y0, A x0, 0): pres dres c.x supp bnd time
         for (long long i = i0 + tid; i < i1; i += BLOCK) {
-          const double a = l.ax[i], rl = l.rl[i], ru = l.ru[i], y = l.y[i];
+          const double a = l.ax[i], rl = l.rl[i], ru = l.ru[i], q = qs(i);
+          const double z = (double)l.y[i] - q;
           part[0] = nanmax(part[0], fabs(fmax(rl - a, 0.0) + fmax(a - ru, 0.0)));
-          part[3] += bsel_row(y, rl, ru) * y;
+          part[3] += bsel_row(z, rl, ru) * z;
+          part[2] += q * a;
         }
         for (long long j = j0 + tid; j < j1; j += BLOCK) {
           const double c = l.c[j], d = c + (double)l.aty[j], lo = l.lo[j], hi = l.hi[j];
@@ -393,15 +444,14 @@ struct PhaseWork {
 
       case PH_A_TRIAL: {    // adx = A dx; y' = prox(y + sigma (ax + 2 adx)); dy.adx, dy.dy
         const T sg = (T)(s.eta * s.omega);
-        struct R { T y, ax, rl, ru; };
+        struct R { T y, ax, rl, ru, q; };
         auto prod = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * l.dx[j]; };
-        auto pre = [&](int i) { return R{l.y[i], l.ax[i], l.rl[i], l.ru[i]}; };
+        auto pre = [&](int i) { return R{l.y[i], l.ax[i], l.rl[i], l.ru[i], qs(i)}; };
         auto epi = [&](int i, const T (&sm)[1], const R& r) {
           const T a = sm[0];
           l.adx[i] = a;
           const T w = r.y + sg * (r.ax + T(2) * a);
-          const T pr = clampT<T>(w / sg, r.rl, r.ru);
-          const T yv = w - sg * pr;
+          const T yv = dual_prox(w, r.q, sg, r.rl, r.ru);
           l.yn[i] = yv;
           const T d = yv - r.y;
           l.dy[i] = d;
@@ -438,16 +488,15 @@ struct PhaseWork {
 
       case PH_A_FUSED: {    // fused dual step (backend.py:260-279) and sum_y += eta y
         const T sg = (T)(s.eta * s.omega), et = (T)s.eta;
-        struct R { T ax, y, rl, ru, sumy; };
+        struct R { T ax, y, rl, ru, sumy, q; };
         auto prod = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * l.dx[j]; };
-        auto pre = [&](int i) { return R{l.ax[i], l.y[i], l.rl[i], l.ru[i], l.sumy[i]}; };
+        auto pre = [&](int i) { return R{l.ax[i], l.y[i], l.rl[i], l.ru[i], l.sumy[i], qs(i)}; };
         auto epi = [&](int i, const T (&sm)[1], const R& r) {
           const T a = sm[0];
           const T axn = r.ax + a;
           l.ax[i] = axn;
           const T w = r.y + sg * (axn + a);
-          const T pr = clampT<T>(w / sg, r.rl, r.ru);
-          const T yv = w - sg * pr;
+          const T yv = dual_prox(w, r.q, sg, r.rl, r.ru);
           l.dy[i] = yv - r.y;
           l.y[i] = yv;
           l.sumy[i] = r.sumy + et * yv;
@@ -533,8 +582,9 @@ struct PhaseWork {
           part[0] += l.c0[j] * v;
         }
         for (long long i = i0 + tid; i < i1; i += BLOCK) {
-          const double v = -(((double)l.y[i] * l.rowf[i]) / s.sobj);
-          l.yo[i] = l.flip ? -v : v;
+          double v = ((double)l.y[i] * l.rowf[i]) / s.sobj;
+          if (l.xacc) v = l.yacc[i] + (s.absorbed ? 0.0 : v / rs);
+          l.yo[i] = l.flip ? v : -v;
         }
       } break;
 
@@ -547,35 +597,34 @@ struct PhaseWork {
         tile_pass<BLOCK, TNNZ, 1, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prod, epi, ldsd);
       } break;
 
-      case PH_REF_X: {      // X += clip(x col, lo_r, hi_r) / Delta
-        const double rs = s.ref_scale;
+      case PH_REF_X: {      // X += clip(x col, lo_r, hi_r) / Delta; Y += (y row / s) / Delta
+        const double rs = s.ref_scale, so = s.sobj;
         for (long long j = j0 + tid; j < j1; j += BLOCK) {
           const double v = fmin(fmax((double)l.x[j] * l.colf[j], l.clo[j]), l.chi[j]);
           l.xacc[j] += v / rs;
         }
+        for (long long i = i0 + tid; i < i1; i += BLOCK)
+          l.yacc[i] += (((double)l.y[i] * l.rowf[i]) / so) / rs;
       } break;
 
-      case PH_REF_KKT: {    // FP64 KKT of (X, y) on the original problem; A X kept
-        const double so = s.sobj;
-        struct RR { double rl, ru, rf, y; };
+      case PH_REF_KKT: {    // FP64 KKT of (X, Y) on the original problem; A X and c + A^T Y kept
+        struct RR { double rl, ru, y; };
         auto prodA = [&](int q, int j, double (&p)[1]) { p[0] = l.av[q] * l.xacc[j]; };
-        auto preA = [&](int i) { return RR{l.rl0[i], l.ru0[i], l.rowf[i], (double)l.y[i]}; };
+        auto preA = [&](int i) { return RR{l.rl0[i], l.ru0[i], l.yacc[i]}; };
         auto epiA = [&](int i, const double (&sm)[1], const RR& r) {
           const double a = sm[0];
           l.axacc[i] = a;
-          const double yu = (r.y * r.rf) / so;
           part[0] = nanmax(part[0], fabs(fmax(r.rl - a, 0.0) + fmax(a - r.ru, 0.0)));
-          part[3] += bsel_row(yu, r.rl, r.ru) * yu;
+          part[3] += bsel_row(r.y, r.rl, r.ru) * r.y;
         };
         tile_pass_pf<BLOCK, TNNZ, 1, false, double>(l.ap, l.ai, l.tilesA, a0, a1, prodA, preA,
                                                     epiA, ldsd);
         struct RC { double c, lo, hi, x; };
-        auto prodT = [&](int q, int i, double (&p)[1]) {
-          p[0] = l.tv[q] * (((double)l.y[i] * l.rowf[i]) / so);
-        };
+        auto prodT = [&](int q, int i, double (&p)[1]) { p[0] = l.tv[q] * l.yacc[i]; };
         auto preT = [&](int j) { return RC{l.c0[j], l.lo0[j], l.hi0[j], l.xacc[j]}; };
         auto epiT = [&](int j, const double (&sm)[1], const RC& r) {
           const double du = r.c + sm[0];
+          l.dacc[j] = du;
           part[0] = nanmax(part[0], fabs(fmax(r.lo - r.x, 0.0) + fmax(r.x - r.hi, 0.0)));
           part[1] = nanmax(part[1], fabs(dres_of(du, r.lo, r.hi)));
           part[2] += r.c * r.x;
@@ -590,8 +639,8 @@ struct PhaseWork {
         const double rs = s.ref_scale;
         for (long long j = j0 + tid; j < j1; j += BLOCK) {
           const double X = l.xacc[j], lo0 = l.lo0[j], hi0 = l.hi0[j], cf = l.colf[j];
-          const double lo = lo0 > -kInf ? (lo0 - X) * rs : -kInf;
-          const double hi = hi0 < kInf ? (hi0 - X) * rs : kInf;
+          const double lo = lo0 > -kInf ? fmax((lo0 - X) * rs, -kTrustRegion) : -kInf;
+          const double hi = hi0 < kInf ? fmin((hi0 - X) * rs, kTrustRegion) : kInf;
           l.wlo[j] = lo;
           l.whi[j] = hi;
           const T los = (T)(lo > -kInf ? lo / cf : -kInf), his = (T)(hi < kInf ? hi / cf : kInf);
@@ -601,20 +650,31 @@ struct PhaseWork {
           l.x[j] = x0;
           l.xr[j] = x0;
           l.sumx[j] = T(0);
+          // objective of the round: Delta (c + A^T Y), scaled as the original
+          const double cj = l.dacc[j] * rs;
+          l.cc[j] = cj;
+          const T cs = (T)((cj * cf) * s.sobj);
+          l.c[j] = cs;
+          l.aty[j] = T(0);      // the round's dual starts at Y (y^ = 0)
+          part[2] += (double)cs * (double)cs;
+          part[3] = nanmax(part[3], fabs(cj));
         }
         for (long long i = i0 + tid; i < i1; i += BLOCK) {
           const double a = l.axacc[i], rl0 = l.rl0[i], ru0 = l.ru0[i], rf = l.rowf[i];
-          const double rl = rl0 > -kInf ? (rl0 - a) * rs : -kInf;
-          const double ru = ru0 < kInf ? (ru0 - a) * rs : kInf;
+          const double rl = rl0 > -kInf ? fmax((rl0 - a) * rs, -kTrustRegion) : -kInf;
+          const double ru = ru0 < kInf ? fmin((ru0 - a) * rs, kTrustRegion) : kInf;
           l.wrl[i] = rl;
           l.wru[i] = ru;
           l.rl[i] = (T)(rl > -kInf ? rl * rf : -kInf);
           l.ru[i] = (T)(ru < kInf ? ru * rf : kInf);
           if (rl > -kInf) part[0] = nanmax(part[0], fabs(rl));
           if (ru < kInf) part[0] = nanmax(part[0], fabs(ru));
-          const double b = (rl > -kInf ? (double)l.rl[i] : 0.0) + (ru < kInf ? (double)l.ru[i] : 0.0);
-          part[1] += b * b;     // ||b'||^2 as the scaling phase forms it
-          l.yr[i] = l.y[i];     // the dual carries over (warm start)
+          // row-slack cost -Delta Y (unscaled and scaled)
+          const double qv = -(l.yacc[i] * rs);
+          l.qc[i] = qv;
+          l.q[i] = (T)((qv * s.sobj) / rf);
+          l.y[i] = T(0);
+          l.yr[i] = T(0);
           l.sumy[i] = T(0);
         }
       } break;
@@ -636,15 +696,17 @@ struct PhaseWork {
     constexpr bool kDerived = sizeof(T) == 8;
 
     auto unscaled_row = [&](int i, double au, double aua, double y, double ya) {
-      const double rf = l.rowf[i], rl0 = l.crl[i], ru0 = l.cru[i];
-      const double yu = (y * rf) / so, yua = (ya * rf) / so;
+      const double rf = l.rowf[i], rl0 = l.crl[i], ru0 = l.cru[i], q = qu(i);
+      const double yu = (y * rf) / so - q, yua = (ya * rf) / so - q;
       v[PUC] = nanmax(v[PUC], fabs(fmax(rl0 - au, 0.0) + fmax(au - ru0, 0.0)));
       v[SUUC] += bsel_row(yu, rl0, ru0) * yu;
+      v[CXUC] += q * au;
       v[PUA] = nanmax(v[PUA], fabs(fmax(rl0 - aua, 0.0) + fmax(aua - ru0, 0.0)));
       v[SUUA] += bsel_row(yua, rl0, ru0) * yua;
+      v[CXUA] += q * aua;
     };
     auto unscaled_col = [&](int j, double atu, double atua, double x, double xa) {
-      const double cf = l.colf[j], lo0 = l.clo[j], hi0 = l.chi[j], c0 = l.c0[j];
+      const double cf = l.colf[j], lo0 = l.clo[j], hi0 = l.chi[j], c0 = cu(j);
       const double xu = x * cf, xua = xa * cf;
       const double du = c0 + atu, dua = c0 + atua;
       v[PUC] = nanmax(v[PUC], fabs(fmax(lo0 - xu, 0.0) + fmax(xu - hi0, 0.0)));
@@ -665,13 +727,16 @@ struct PhaseWork {
     auto epiA = [&](int i, const T (&sm)[2]) {
       l.axf[i] = sm[0];
       l.axa[i] = sm[1];
-      const double rl = l.rl[i], ru = l.ru[i], y = l.y[i];
+      const double rl = l.rl[i], ru = l.ru[i], y = l.y[i], q = qs(i);
       const double ya = (T)((double)l.sumy[i] / inv_se);
       const double am = l.ax[i], ag = sm[1];
+      const double z = y - q, za = ya - q;
       v[PC] = nanmax(v[PC], fabs(fmax(rl - am, 0.0) + fmax(am - ru, 0.0)));
-      v[SUC] += bsel_row(y, rl, ru) * y;
+      v[SUC] += bsel_row(z, rl, ru) * z;
+      v[CXC] += q * am;
       v[PA] = nanmax(v[PA], fabs(fmax(rl - ag, 0.0) + fmax(ag - ru, 0.0)));
-      v[SUA] += bsel_row(ya, rl, ru) * ya;
+      v[SUA] += bsel_row(za, rl, ru) * za;
+      v[CXA] += q * ag;
       if (kDerived) {
         const double rf = l.rowf[i];
         unscaled_row(i, (double)sm[0] / rf, (double)sm[1] / rf, y, ya);
@@ -807,15 +872,18 @@ struct Transition {
           const double g = exp2(rint(log2(sqrt(t[0] * (-t[1])))));
           if (g > 0.0 && g <= 1.79769313486231570e308) s.sobj = 1.0 / g;
         }
+        s.sobj0 = s.sobj;
         return PH_SCALE;
       }
       case PH_SCALE: {
         s.ptol0 = fmin(P.eps_abs + P.eps_rel * t[4], P.feas_cap);
         s.dtol0 = fmin(P.eps_abs + P.eps_rel * t[3], P.feas_cap);
         s.rhs0 = t[4];
+        s.cinf0 = t[3];
         s.ref_scale = 1.0;
         s.round = 0;
         s.absorbed = 0;
+        s.conv_last = 0;
         s.k_done = 0;
         s.kmax = P.max_iter;
         if (refining()) {
@@ -828,7 +896,6 @@ struct Transition {
           s.dtol = s.dtol0;
         }
         const double cn = sqrt(t[0]), bn = sqrt(t[1]);
-        s.cnorm = cn;
         s.nv = sqrt(t[2]);
         const double w = bn > 0.0 ? exp(log(fmax(cn, 1e-12)) - log(fmax(bn, 1e-12))) : 1.0;
         s.omega = fmin(fmax(w, 1e-4), 1e4);
@@ -915,9 +982,24 @@ struct Transition {
         const double gu = fabs(pou - dou);
         s.use_avg = use_avg;
         s.primal_ready = 0;
+        if (refining() && s.round > 0 && s.conv_last &&
+            s.k >= (4 * s.k_round0 > 2000 ? 4 * s.k_round0 : 2000)) {
+          // a correction round that does not settle within its budget: X
+          // already met _converged, so report X and drop the round
+          s.status = ST_OPTIMAL;
+          s.absorbed = 1;
+          return PH_FINAL;
+        }
         const double ea = refining() ? P.refine_eps : P.eps_abs;
         const double er = refining() ? P.refine_eps : P.eps_rel;
-        if (pu <= s.ptol && du <= s.dtol && gu <= ea + er * (fabs(pou) + fabs(dou))) {
+        // a correction round's gap is Delta^2 times the original gap at X + x^ /
+        // Delta, Y + y^ / Delta, while its objective is the (tiny) change in
+        // objective: its gap is measured against Delta^2 x the last original
+        // gap, i.e. the round must shrink the original gap by the tolerance
+        double gscale = fabs(pou) + fabs(dou);
+        if (refining() && s.round > 0)
+          gscale = fmax(gscale, s.ref_scale * s.ref_scale * s.gap_last);
+        if (pu <= s.ptol && du <= s.dtol && gu <= ea + er * gscale) {
           if (refining()) return use_avg ? PH_COPY_AVG : PH_REF_X;   // round converged
           s.status = ST_OPTIMAL;
           s.fin[0] = pu; s.fin[1] = du; s.fin[2] = gu; s.fin[3] = pou; s.fin[4] = dou;
@@ -965,14 +1047,19 @@ struct Transition {
           h[0] = (double)(s.k_done + s.k); h[1] = pres; h[2] = dres; h[3] = gap; h[4] = pou;
           s.ref_rows = s.round + 1;
         }
+        if (s.round == 0) s.k_round0 = s.k;
         s.k_done += s.k;
         s.k = 0;
         const bool conv = pres <= s.ptol0 && dres <= s.dtol0 &&
                           gap <= P.eps_abs + P.eps_rel * (fabs(pou) + fabs(dou));
+        s.conv_last = conv;
+        s.gap_last = gap;
+        if (conv) {
+          s.fin[0] = pres; s.fin[1] = dres; s.fin[2] = gap; s.fin[3] = pou; s.fin[4] = dou;
+        }
         const bool target = pres <= P.refine_tol * (1.0 + s.rhs0);
         if (conv && (target || s.round + 1 >= P.refine_rounds)) {
           s.status = ST_OPTIMAL;
-          s.fin[0] = pres; s.fin[1] = dres; s.fin[2] = gap; s.fin[3] = pou; s.fin[4] = dou;
           return PH_FINAL;
         }
         if (s.k_done >= P.max_iter) {
@@ -987,11 +1074,19 @@ struct Transition {
           s.status = ST_NUMERICAL;   // the rounds could not reach the tolerance
           return PH_FINAL;
         }
-        // Delta: the power of two nearest below 1 / residual, at most 2^20
-        // times the last one (and never below 1)
-        double d = exp2(floor(log2(1.0 / fmax(pres, 1e-300))));
-        d = fmin(d, s.ref_scale * 1048576.0);
-        s.ref_scale = fmax(d, 1.0);
+        // Delta: the power of two nearest below 1 / error, at most 2^20 times
+        // the last one (and never below 1).  The error is the larger of the
+        // residuals and the gap over the cost scale, which estimates how far X
+        // is from optimal: a zoom set by the primal residual alone would trap
+        // a feasible, suboptimal X inside the trust region.
+        const double err = fmax(fmax(pres, dres), gap / (1.0 + s.cinf0));
+        double d = exp2(floor(log2(1.0 / fmax(err, 1e-300))));
+        d = fmax(fmin(d, s.ref_scale * 1048576.0), 1.0);
+        // the round works in x^ = Delta (x - X), y^ = Delta (y - Y) with
+        // objective Delta (c + A^T Y) and bounds scaled by Delta: the same
+        // PDHG in zoomed coordinates, so step size, primal weight and the
+        // objective scale carry over unchanged
+        s.ref_scale = d;
         ++s.round;
         return PH_REF_SHIFT;
       }
@@ -999,12 +1094,7 @@ struct Transition {
         // tolerances of the new round: relative to its own right-hand side,
         // capped at the original problem's scale
         s.ptol = fmin(P.refine_eps + P.refine_eps * t[0], P.refine_eps * (1.0 + s.rhs0));
-        // step size and primal weight start as for a new problem (pdlp.py:
-        // 289-296): the primal is rescaled by Delta, the dual is not
-        const double bn = sqrt(t[1]);
-        const double w = bn > 0.0 ? exp(log(fmax(s.cnorm, 1e-12)) - log(fmax(bn, 1e-12))) : 1.0;
-        s.omega = fmin(fmax(w, 1e-4), 1e4);
-        s.eta = 1.0 / s.anorm;
+        s.dtol = fmin(P.refine_eps + P.refine_eps * t[3], P.refine_eps * (1.0 + s.cinf0));
         s.absorbed = 0;
         s.kmax = P.max_iter - s.k_done;
         return PH_AX0;

@@ -217,6 +217,11 @@ void carve_lp(Carver& cv, LpDev<T>& L, int m, int n, long long nnz, long long hi
     L.whi = cv.take<double>(n);
     L.xacc = cv.take<double>(n);
     L.axacc = cv.take<double>(m);
+    L.yacc = cv.take<double>(m);
+    L.qc = cv.take<double>(m);
+    L.dacc = cv.take<double>(n);
+    L.cc = cv.take<double>(n);
+    L.q = reinterpret_cast<T*>(cv.take<float>(m));
   }
 }
 
@@ -734,7 +739,7 @@ void fill_info(const LpState& t, sovlp_info* f) {
   f->step_size = t.eta;
   f->primal_weight = t.omega;
   f->objective = t.objective;
-  f->scale_obj = t.sobj;
+  f->scale_obj = t.sobj0;
   f->device_ticks = t.ticks;
   f->iterations = t.k_done + t.k;
   f->refine_rows = t.ref_rows;

@@ -52,8 +52,9 @@ class PDLPParams:
     precision: int = 64
     team: str = "auto"
     chunk_iters: int = 8192
-    # iterative refinement of the FP32 engine (rounds: None = none; each
-    # round to relative KKT max(eps_rel, 1e-6))
+    # iterative refinement of the FP32 engine (None: 8 rounds at precision
+    # 32, none at 64; each round to relative KKT max(eps_rel, 1e-4), which
+    # FP32 iterates reach; the rounds together reach eps and feas_cap)
     refine_rounds: int | None = None
     refine_eps: float | None = None
     refine_tol: float = 1e-9
@@ -62,8 +63,8 @@ class PDLPParams:
         prec = int(self.precision if precision is None else precision)
         rounds = self.refine_rounds
         if rounds is None:
-            rounds = 0
-        eps = self.refine_eps if self.refine_eps is not None else max(self.eps_rel, 1e-6)
+            rounds = 8 if prec == 32 else 0
+        eps = self.refine_eps if self.refine_eps is not None else max(self.eps_rel, 1e-4)
         return int(rounds), float(eps)
 
     def to_c(self, precision: int | None = None) -> _lib.sovlp_params:

@@ -52,14 +52,39 @@ def test_result_is_identical_alone_and_in_the_mix():
     assert np.array_equal(s.x, mixed.solutions[0].x)
 
 
-def test_fp32_batch_reaches_loose_tolerance():
-    print("[gpu] fp32 batch", flush=True)
-    kos = _batch(8, base=4000)
-    par = PDLPParams(eps_abs=1e-4, eps_rel=1e-4, feas_cap=1e-3)
+def test_fp32_refined_batch_reaches_closed_form_to_1e8():
+    """FP32 iterates + FP64 reductions + iterative refinement: every LP meets
+    _converged at 1e-6 (feas_cap 1e-6) and its objective is within 1e-8 of the
+    closed-form optimum; each refinement round's record is reported."""
+    print("[gpu] fp32 + refinement batch of 10", flush=True)
+    kos = _batch(10, base=4000)
+    par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6)
     res = solve_batch([k.problem for k in kos], par, precision=32, block=256)
+    print(f"[gpu] fp32 batch done in {res.seconds:.3f}s", flush=True)
     for ko, s in zip(kos, res.solutions):
-        assert s.status == Status.OPTIMAL, (ko.problem.name, s)
-        assert abs(s.objective - ko.f_star) <= 1e-3 * max(1.0, abs(ko.f_star))
+        assert s.status == Status.OPTIMAL, (ko.problem.name, s.status, s.info.refinement)
+        assert abs(s.objective - ko.f_star) <= 1e-8 * max(1.0, abs(ko.f_star)), \
+            (ko.problem.name, s.objective, ko.f_star, s.info.refinement)
+        rv, bv, _ = ko.problem.violation(s.x)
+        assert rv <= 1e-6 and bv <= 1e-6
+        ref = s.info.refinement
+        assert len(ref) >= 1 and ref[-1][1] <= 1e-6
+        # the last row is the accepted point: its objective is the reported one
+        assert abs(ref[-1][4] - s.objective) <= 1e-9 * max(1.0, abs(s.objective))
+        # textbook duals on output, as in FP64
+        np.testing.assert_allclose(s.reduced_costs, ko.problem.c - ko.problem.A.rmatvec(s.y),
+                                   rtol=1e-9, atol=1e-9)
+
+
+def test_fp32_result_is_identical_alone_and_in_the_mix():
+    print("[gpu] fp32 alone vs mixed", flush=True)
+    kos = _batch(5, base=6000)
+    par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6)
+    mixed = solve_batch([k.problem for k in kos], par, precision=32, block=256)
+    alone = solve_batch([kos[2].problem], par, precision=32, block=256)
+    a, b = alone.solutions[0], mixed.solutions[2]
+    assert a.iterations == b.iterations
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
 
 
 def test_small_workgroups_agree_with_large():

@@ -113,3 +113,33 @@ def test_grid_team_on_a_medium_lp():
         assert abs(s.objective - ko.f_star) <= 1e-5 * max(1.0, abs(ko.f_star))
         rv, bv, _ = ko.problem.violation(s.x)
         assert rv <= 1e-6 and bv <= 1e-12
+
+
+def test_fp32_refined_single_lp_matches_fp64_oracle_to_1e8():
+    """One LP through the single-LP engine in FP32 with refinement: the
+    objective is within 1e-8 of the FP64 oracle and of the closed form, and
+    the per-round errors shrink."""
+    print("[gpu] small_lp 500x2000 fp32 + refinement", flush=True)
+    ko = small_lp(0)
+    par = PDLPParams(device="gpu", precision=32, eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6)
+    s = solve_pdlp(ko.problem, par)
+    r = oracle.solve(ko.problem)                      # FP64 oracle at 1e-8
+    assert s.status == Status.OPTIMAL, s.info.refinement
+    for ref_obj in (ko.f_star, r.objective):
+        assert abs(s.objective - ref_obj) <= 1e-8 * max(1.0, abs(ref_obj))
+    rows = s.info.refinement
+    assert len(rows) >= 2
+    errs = [abs(row[4] - ko.f_star) / max(1.0, abs(ko.f_star)) for row in rows]
+    print("[gpu] refinement rounds:", [(row[0], f"{row[1]:.1e}", f"{e:.1e}")
+                                       for row, e in zip(rows, errs)], flush=True)
+    assert rows[-1][1] < rows[0][1]
+    rv, bv, _ = ko.problem.violation(s.x)
+    assert rv <= 1e-6 and bv <= 1e-6
+
+
+def test_fp32_refined_maximisation_and_free_variables():
+    for name, expect in (("toy_max", 11.0), ("testprob", 16.0)):
+        p = _problem(next(c for c in GOLD["cases"] if c["name"] == name))
+        s = solve_pdlp(p, PDLPParams(device="gpu", precision=32, eps_abs=1e-6, eps_rel=1e-6))
+        assert s.status == Status.OPTIMAL and abs(s.objective - expect) < 1e-8 * max(1, expect)
+        np.testing.assert_allclose(s.reduced_costs, p.c - p.A.rmatvec(s.y), rtol=1e-9, atol=1e-9)
