@@ -99,6 +99,8 @@ struct LpState {             // everything the loop carries between phases
   long long k_done;          // iterations of finished rounds
   long long kmax;            // iteration limit of the current round
   long long k_round0;        // iterations of round 0
+  long long k_best;          // iteration of the round's last KKT-ratio halving
+  double kkt_best;           // best KKT ratio of the round (stall detection)
   int phase, status, sweep, pit, trial, primal_ready, time_flag, use_avg, next_mode;
   int round;                 // refinement round (0: the plain solve)
   int conv_last;             // the last X evaluated satisfied _converged
@@ -933,6 +935,8 @@ struct Transition {
         s.sum_eta = 0.0;
         s.k = s.epoch = s.attempts = 0;
         if (s.round == 0) s.restarts = s.rejected = s.hist_len = 0;
+        s.kkt_best = __builtin_inf();
+        s.k_best = 0;
         s.status = ST_RUNNING;
         s.time_flag = t[5] > 0.0;
         s.primal_ready = 0;
@@ -982,14 +986,6 @@ struct Transition {
         const double gu = fabs(pou - dou);
         s.use_avg = use_avg;
         s.primal_ready = 0;
-        if (refining() && s.round > 0 && s.conv_last &&
-            s.k >= (4 * s.k_round0 > 2000 ? 4 * s.k_round0 : 2000)) {
-          // a correction round that does not settle within its budget: X
-          // already met _converged, so report X and drop the round
-          s.status = ST_OPTIMAL;
-          s.absorbed = 1;
-          return PH_FINAL;
-        }
         const double ea = refining() ? P.refine_eps : P.eps_abs;
         const double er = refining() ? P.refine_eps : P.eps_rel;
         // a correction round's gap is Delta^2 times the original gap at X + x^ /
@@ -999,7 +995,29 @@ struct Transition {
         double gscale = fabs(pou) + fabs(dou);
         if (refining() && s.round > 0)
           gscale = fmax(gscale, s.ref_scale * s.ref_scale * s.gap_last);
-        if (pu <= s.ptol && du <= s.dtol && gu <= ea + er * gscale) {
+        const double gtol = ea + er * gscale;
+        if (refining()) {
+          // a round stalls when its KKT ratio has not halved for 4096
+          // iterations (FP32 iterates can stall short of the round's
+          // tolerance), or after 32768.  If the last X met _converged, X is
+          // reported and the round dropped; otherwise the solve refines from
+          // the round's current candidate (a round that then cuts the gap by
+          // less than 10x makes the next one zoom out)
+          const double ratio = fmax(fmax(pu / s.ptol, du / s.dtol), gu / gtol);
+          if (ratio < 0.5 * s.kkt_best) {
+            s.kkt_best = ratio;
+            s.k_best = s.k;
+          }
+          if (ratio > 1.0 && (s.k - s.k_best >= 4096 || s.k >= 32768)) {
+            if (s.round > 0 && s.conv_last) {
+              s.absorbed = 1;
+              s.status = ST_OPTIMAL;
+              return PH_FINAL;
+            }
+            return use_avg ? PH_COPY_AVG : PH_REF_X;
+          }
+        }
+        if (pu <= s.ptol && du <= s.dtol && gu <= gtol) {
           if (refining()) return use_avg ? PH_COPY_AVG : PH_REF_X;   // round converged
           s.status = ST_OPTIMAL;
           s.fin[0] = pu; s.fin[1] = du; s.fin[2] = gu; s.fin[3] = pou; s.fin[4] = dou;
@@ -1042,16 +1060,17 @@ struct Transition {
         const double pres = t[0], dres = t[1];
         const double pou = t[2] + offset0, dou = (-t[3] + t[4]) + offset0;
         const double gap = fabs(pou - dou);
-        if (s.round < kMaxRounds) {
-          double* h = rhist + 5 * s.round;
+        if (s.ref_rows < kMaxRounds) {
+          double* h = rhist + 5 * s.ref_rows;
           h[0] = (double)(s.k_done + s.k); h[1] = pres; h[2] = dres; h[3] = gap; h[4] = pou;
-          s.ref_rows = s.round + 1;
+          ++s.ref_rows;
         }
         if (s.round == 0) s.k_round0 = s.k;
         s.k_done += s.k;
         s.k = 0;
         const bool conv = pres <= s.ptol0 && dres <= s.dtol0 &&
                           gap <= P.eps_abs + P.eps_rel * (fabs(pou) + fabs(dou));
+        const double gap_before = s.gap_last;
         s.conv_last = conv;
         s.gap_last = gap;
         if (conv) {
@@ -1081,7 +1100,11 @@ struct Transition {
         // a feasible, suboptimal X inside the trust region.
         const double err = fmax(fmax(pres, dres), gap / (1.0 + s.cinf0));
         double d = exp2(floor(log2(1.0 / fmax(err, 1e-300))));
-        d = fmax(fmin(d, s.ref_scale * 1048576.0), 1.0);
+        d = fmin(d, s.ref_scale * 1048576.0);
+        // a round that cut the gap by less than 10x was held back by its trust
+        // region: zoom out 16x so the next one can move X further
+        if (s.round > 0 && gap > 0.1 * gap_before) d = s.ref_scale / 16.0;
+        d = fmax(d, 1.0);
         // the round works in x^ = Delta (x - X), y^ = Delta (y - Y) with
         // objective Delta (c + A^T Y) and bounds scaled by Delta: the same
         // PDHG in zoomed coordinates, so step size, primal weight and the
