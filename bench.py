@@ -64,6 +64,22 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int):
+    """HBM bytes per launch of ``kernel`` on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's reading;
+    see profiles/), or None when they were taken on another configuration."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_pdlp_run_kernel_fp32_4096_lps.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    want = f"{lps} uneven LPs, seeds {seed}..{seed + lps - 1}, " \
+           f"{'FP32 + refinement' if precision == 32 else 'FP64'}, block {block}"
+    if d.get("kernel") != kernel or d.get("workload") != want:
+        return None
+    return float(d["traffic_bytes_per_launch"])
+
+
 def plain_fp64_rate(problems, tol, sample):
     """LPs/s of the single-LP FP64 engine, one LP at a time, inputs resident."""
     import torch
@@ -204,7 +220,7 @@ def main():
         "mean_refinement_rounds": float(np.mean([len(s.info.refinement) for s in sols])),
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": measured_traffic(kernel, a.lps, a.seed, a.precision, a.block),
                      "algorithmic_bytes_per_launch": step_bytes / graphs[-1],
                      "avg_launch_us": launch_ms * 1e3, "launches": launches},
     }
