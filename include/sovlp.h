@@ -126,7 +126,8 @@ int sovlp_spmv_tiled_f32(int rows, const int32_t *ptr32, const int32_t *idx, con
  *                X += x^ / Delta.  The dual carries over between rounds.
  * refine_eps     relative tolerance (eps_abs = eps_rel) of every round
  * refine_tol     the loop also stops once the primal residual of X is
- *                <= refine_tol (1 + ||finite row bounds||_inf), after
+ *                <= refine_tol (1 + ||finite row bounds||_inf) and the duality
+ *                gap of X, Y is <= refine_tol (1 + |pobj| + |dobj|), after
  *                _converged (pdlp.py:523-530) holds for X on the original. */
 typedef struct {
   double eps_abs, eps_rel, feas_cap;
@@ -271,6 +272,75 @@ int sovlp_batch_refinement(sovlp_batch *h, int index, double **rows);
 int sovlp_batch_phase_counts(sovlp_batch *h, int64_t *counts, sovlp_stream_t stream);
 
 int sovlp_batch_destroy(sovlp_batch *h);
+
+/* ---- serving: continuous batching over a slot table -------------------
+ * No reference counterpart: the reference solves one LP per call
+ * (src/sovopt/lp/pdlp.py:238) from a cold start (x0 = clip(0), y0 = 0,
+ * pdlp.py:274-275).  A server holds `slots` LP slots that share every launch
+ * of one recorded graph (one workgroup per slot, up to `phases_per_launch`
+ * phases each, as the batched engine).  LPs are admitted into free slots
+ * between launches, so requests that arrive while others run join the next
+ * launch; each LP's arithmetic depends only on its own data and the
+ * workgroup size, so its result is the same alone and in any mix.  The
+ * caller owns all device memory: the server's table (workspace_bytes) and one
+ * workspace per admitted LP (lp_bytes), which must stay allocated until the
+ * slot is released.  A slot's tile plans use sovlp_batch_geometry's sizes for
+ * the server's precision and block.  One host thread per server. */
+typedef struct sovlp_server sovlp_server;
+
+/* Bytes of the server's own device table for `slots` slots. */
+int sovlp_server_workspace_bytes(int slots, int precision, int64_t *bytes);
+
+/* Workspace bytes of one admitted LP (history_cap history rows). */
+int sovlp_server_lp_bytes(const sovlp_lp *lp, int precision, int64_t history_cap,
+                          int64_t *bytes);
+
+/* Creates an empty server (every slot free) and records its launch graph.
+ * `block` 64 or 256; params->time_limit is each LP's device time budget,
+ * counted from its first phase.  Synchronises. */
+int sovlp_server_create(int slots, const sovlp_params *params, int precision, int block,
+                        int phases_per_launch, void *workspace, int64_t workspace_bytes,
+                        sovlp_stream_t stream, sovlp_server **out);
+
+/* Admits an LP into free slot `slot` (asynchronous, stream-ordered before
+ * the next step).  Warm start: x0 (n) and y0 (m) are device vectors in the
+ * units and signs of a report (sovlp_server_outputs of an earlier solve of an
+ * LP with the same matrix shape), omega0 > 0 a primal weight to start from;
+ * NULL / 0 give the reference's cold start.  x0 and y0 must stay allocated
+ * until the slot is released. */
+int sovlp_server_admit(sovlp_server *h, int slot, const sovlp_lp *lp, const double *x0,
+                       const double *y0, double omega0, void *workspace,
+                       int64_t workspace_bytes, int64_t history_cap, sovlp_stream_t stream);
+
+/* Replays the launch graph `launches` times (0: only poll), then copies the
+ * slot states to the host.  finished[slots] (optional) receives 1 for every
+ * occupied slot whose LP has terminated, *running the occupied slots still
+ * running.  Synchronises. */
+int sovlp_server_step(sovlp_server *h, int launches, sovlp_stream_t stream, int32_t *finished,
+                      int32_t *running);
+
+/* The slot's engine state as of the last step. */
+int sovlp_server_info(sovlp_server *h, int slot, sovlp_info *info);
+
+/* Device pointers of the slot's report (x, y, reduced costs; FP64, original
+ * units), valid once the slot has finished and until it is released. */
+int sovlp_server_outputs(sovlp_server *h, int slot, double **x, double **y, double **d);
+
+/* Device pointer of the slot's refinement record (as sovlp_pdlp_refinement). */
+int sovlp_server_refinement(sovlp_server *h, int slot, double **rows);
+
+/* counts[sovlp_phase_slots()] of the slot's current LP; synchronises. */
+int sovlp_server_phase_counts(sovlp_server *h, int slot, int64_t *counts,
+                              sovlp_stream_t stream);
+
+/* Frees a finished slot for the next admission (its LP's memory may then be
+ * reused).  Fails while the slot's LP is still running. */
+int sovlp_server_release(sovlp_server *h, int slot);
+
+/* Device clock rate of the server's device in Hz (for tick counts). */
+int sovlp_server_clock_hz(sovlp_server *h, double *hz);
+
+int sovlp_server_destroy(sovlp_server *h);
 
 #ifdef __cplusplus
 }

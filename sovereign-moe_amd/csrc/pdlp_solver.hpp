@@ -146,6 +146,11 @@ struct LpDev {
   LpState* st;
   long long* pcount;         // [kPhaseSlots] phase counters
   double* partials;          // [W][kMaxRed] per-workgroup partial sums (W-way phases)
+  // warm start (served resolves; null / 0 = the reference's cold start
+  // x0 = clip(0), y0 = 0, omega0 from the norms): x and y in the units and
+  // signs of the report (xo, yo), and a primal weight to start from
+  const double *wx0, *wy0;
+  double womega;
 };
 
 __device__ __forceinline__ double pow2_round(double s) {
@@ -354,7 +359,8 @@ struct PhaseWork {
           l.c[j] = cj;
           l.lo[j] = lo;
           l.hi[j] = hi;
-          const T x0 = clampT<T>(T(0), lo, hi);
+          // warm start: the reported x back in scaled units (x' = x / col)
+          const T x0 = clampT<T>(l.wx0 ? (T)(l.wx0[j] / cf) : T(0), lo, hi);
           l.x[j] = x0;
           l.xr[j] = x0;
           l.sumx[j] = T(0);
@@ -377,20 +383,35 @@ struct PhaseWork {
           }
           l.rl[i] = rl;
           l.ru[i] = ru;
-          l.y[i] = T(0);
-          l.yr[i] = T(0);
+          // warm start: the reported (textbook) y back to the internal sign
+          // and scaled units, inverting FINAL: y' = -y s / row (y' = y s / row
+          // for a maximisation)
+          T y0 = T(0);
+          if (l.wy0) {
+            const double yw = l.flip ? l.wy0[i] : -l.wy0[i];
+            y0 = (T)((yw * so) / rf);
+          }
+          l.y[i] = y0;
+          l.yr[i] = y0;
           l.sumy[i] = T(0);
           const double b = (rl > (T)(-kInf) ? (double)rl : 0.0) + (ru < (T)kInf ? (double)ru : 0.0);
           part[1] += b * b;
           if (rl0 > -kInf) part[4] = nanmax(part[4], fabs(rl0));
           if (ru0 < kInf) part[4] = nanmax(part[4], fabs(ru0));
         }
+        // a warm start's primal weight reaches the transition as a sum of one term
+        if (rank == 0 && tid == 0 && l.womega > 0.0) part[5] = l.womega;
       } break;
 
-      case PH_AX0: {        // ax = A x0; v = v0 / ||v0|| (kept in xn)
+      case PH_AX0: {        // ax = A x0 (aty = A^T y0 when warm); v = v0 / ||v0|| (in xn)
         auto prod = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * l.x[j]; };
         auto epi = [&](int i, const T (&sm)[1]) { l.ax[i] = sm[0]; };
         tile_pass<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, epi, lds);
+        if (l.wy0 && s.round == 0) {
+          auto prodT = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * l.y[i]; };
+          auto epiT = [&](int j, const T (&sm)[1]) { l.aty[j] = sm[0]; };
+          tile_pass<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prodT, epiT, lds);
+        }
         const double nv = s.nv;
         if (nv != 0.0)
           for (long long j = j0 + tid; j < j1; j += BLOCK) l.xn[j] = (T)(l.v0[j] / nv);
@@ -901,6 +922,9 @@ struct Transition {
         s.nv = sqrt(t[2]);
         const double w = bn > 0.0 ? exp(log(fmax(cn, 1e-12)) - log(fmax(bn, 1e-12))) : 1.0;
         s.omega = fmin(fmax(w, 1e-4), 1e4);
+        // warm start: the primal weight the earlier solve ended with, in the
+        // range the primal-weight update keeps it in (pdlp.py:459-461)
+        if (t[5] > 0.0) s.omega = fmin(fmax(t[5], 1e-6), 1e6);
         return PH_AX0;
       }
       case PH_AX0:
@@ -1076,7 +1100,10 @@ struct Transition {
         if (conv) {
           s.fin[0] = pres; s.fin[1] = dres; s.fin[2] = gap; s.fin[3] = pou; s.fin[4] = dou;
         }
-        const bool target = pres <= P.refine_tol * (1.0 + s.rhs0);
+        // refinement target: primal residual and duality gap of X, Y both at
+        // refine_tol relative, so the objective is that close to the optimum
+        const bool target = pres <= P.refine_tol * (1.0 + s.rhs0) &&
+                            gap <= P.refine_tol * (1.0 + fabs(pou) + fabs(dou));
         if (conv && (target || s.round + 1 >= P.refine_rounds)) {
           s.status = ST_OPTIMAL;
           return PH_FINAL;

@@ -1030,3 +1030,287 @@ int sovlp_batch_destroy(sovlp_batch* h) {
 }
 
 }  // extern "C"
+
+// ====================================================================== server
+// A slot table for continuous batching: `slots` LPs share every launch of one
+// recorded graph (one workgroup per slot, up to `phases` phases each).  An
+// LP is admitted into a free slot between launches (its descriptor and its
+// fresh state are copied in stream order), runs until its state reaches
+// DONE, and is released by the host after the result is read.  Empty and
+// finished slots point at a DONE state in the server's own workspace, so
+// their workgroups return at once and never touch a released LP's memory.
+struct sovlp_server {
+  int slots, precision, block, phases;
+  void* lps;                 // LpDev<T>[slots] (device)
+  LpState* states;           // [slots] (device)
+  long long* pcount;         // [slots][kPhaseSlots] (device)
+  StepCtl* ctl;
+  unsigned* done;
+  char* hstage;              // pinned LpDev<T>[slots]: descriptors being copied in
+  LpState* hinit;            // pinned [slots]: fresh states being copied in
+  LpState* hsnap;            // pinned [slots]: states after the last step
+  std::vector<char> busy, seen_done;
+  std::vector<double*> xo, yo, dout, rhist;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  hipStream_t cap;
+  double hz;
+};
+
+namespace {
+
+template <class T>
+size_t server_ws_T(int slots) {
+  Carver cv{nullptr, 0, 0};
+  cv.take<LpDev<T>>(slots);
+  cv.take<LpState>(slots);
+  cv.take<long long>((size_t)slots * kPhaseSlots);
+  cv.take<StepCtl>(1);
+  cv.take<unsigned>(64);
+  return cv.off + 256;
+}
+
+template <class T>
+size_t server_lp_ws_T(const sovlp_lp* lp, long long hist_cap) {
+  Carver cv{nullptr, 0, 0};
+  LpDev<T> L{};
+  carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
+  return cv.off + 256;
+}
+
+template <class T>
+int server_create_T(int slots, const sovlp_params* prm, int block, int phases, void* ws,
+                    int64_t ws_bytes, hipStream_t st, sovlp_server* h) {
+  if ((size_t)ws_bytes < server_ws_T<T>(slots)) return fail(1, "workspace too small");
+  Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
+  LpDev<T>* d_lps = cv.take<LpDev<T>>(slots);
+  LpState* states = cv.take<LpState>(slots);
+  long long* pcount = cv.take<long long>((size_t)slots * kPhaseSlots);
+  StepCtl* ctl = cv.take<StepCtl>(1);
+  unsigned* done = cv.take<unsigned>(64);
+  h->slots = slots;
+  h->precision = sizeof(T) == 8 ? 64 : 32;
+  h->block = block;
+  h->phases = phases;
+  h->lps = d_lps;
+  h->states = states;
+  h->pcount = pcount;
+  h->ctl = ctl;
+  h->done = done;
+  h->busy.assign(slots, 0);
+  h->seen_done.assign(slots, 0);
+  h->xo.assign(slots, nullptr);
+  h->yo.assign(slots, nullptr);
+  h->dout.assign(slots, nullptr);
+  h->rhist.assign(slots, nullptr);
+  HIPCHK(hipHostMalloc((void**)&h->hstage, sizeof(LpDev<T>) * (size_t)slots, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&h->hinit, sizeof(LpState) * (size_t)slots, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&h->hsnap, sizeof(LpState) * (size_t)slots, hipHostMallocDefault));
+  LpDev<T>* stage = reinterpret_cast<LpDev<T>*>(h->hstage);
+  for (int i = 0; i < slots; ++i) {
+    stage[i] = LpDev<T>{};
+    stage[i].st = states + i;
+    stage[i].pcount = pcount + (size_t)i * kPhaseSlots;
+    h->hinit[i] = LpState{};
+    h->hinit[i].phase = PH_DONE;
+    h->hinit[i].status = ST_RUNNING;
+  }
+  StepCtl c0{};
+  std::memcpy(&c0.P, prm, sizeof(Params));
+  const double secs = prm->time_limit > 0.0 ? prm->time_limit : 0.0;
+  double hz = 0.0;
+  if (int rc = clock_hz(&hz)) return rc;
+  h->hz = hz;
+  c0.ticks = (long long)std::min(secs * hz, 9.0e18);
+  HIPCHK(hipMemcpyAsync(d_lps, stage, sizeof(LpDev<T>) * (size_t)slots, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(states, h->hinit, sizeof(LpState) * (size_t)slots, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(pcount, 0, sizeof(long long) * kPhaseSlots * (size_t)slots, st));
+  HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(done, 0, sizeof(unsigned), st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+  HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+  if (block == 64)
+    pdlp_run_kernel<T, 64, kTnnzSmall><<<slots, 64, 0, h->cap>>>(d_lps, ctl, done, phases);
+  else
+    pdlp_run_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>
+        <<<slots, kBlock, 0, h->cap>>>(d_lps, ctl, done, phases);
+  HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
+  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+template <class T>
+int server_admit_T(sovlp_server* h, int slot, const sovlp_lp* lp, const double* x0,
+                   const double* y0, double omega0, void* ws, int64_t ws_bytes,
+                   int64_t hist_cap, hipStream_t st) {
+  if ((size_t)ws_bytes < server_lp_ws_T<T>(lp, hist_cap)) return fail(1, "LP workspace too small");
+  Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
+  LpDev<T> L{};
+  carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
+  fill_lp<T>(L, *lp);
+  L.st = h->states + slot;   // the state lives in the server's table, not the LP's workspace
+  L.pcount = h->pcount + (size_t)slot * kPhaseSlots;
+  L.wx0 = x0;
+  L.wy0 = y0;
+  L.womega = omega0 > 0.0 ? omega0 : 0.0;
+  LpDev<T>* stage = reinterpret_cast<LpDev<T>*>(h->hstage);
+  stage[slot] = L;
+  h->hinit[slot] = LpState{};
+  h->hinit[slot].phase = PH_INIT;
+  h->hinit[slot].status = ST_RUNNING;
+  LpDev<T>* d_lps = reinterpret_cast<LpDev<T>*>(h->lps);
+  HIPCHK(hipMemcpyAsync(d_lps + slot, &stage[slot], sizeof(LpDev<T>), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(h->states + slot, &h->hinit[slot], sizeof(LpState), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(h->pcount + (size_t)slot * kPhaseSlots, 0, sizeof(long long) * kPhaseSlots, st));
+  h->xo[slot] = L.xo;
+  h->yo[slot] = L.yo;
+  h->dout[slot] = L.dout;
+  h->rhist[slot] = L.rhist;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sovlp_server_workspace_bytes(int slots, int precision, int64_t* bytes) {
+  if (slots < 1 || !bytes) return fail(1, "slots must be >= 1");
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  *bytes = (int64_t)(precision == 64 ? server_ws_T<double>(slots) : server_ws_T<float>(slots));
+  return 0;
+}
+
+int sovlp_server_lp_bytes(const sovlp_lp* lp, int precision, int64_t history_cap,
+                          int64_t* bytes) {
+  if (!lp || !bytes) return fail(1, "null argument");
+  if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) return fail(1, "negative dimension");
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  *bytes = (int64_t)(precision == 64 ? server_lp_ws_T<double>(lp, history_cap)
+                                     : server_lp_ws_T<float>(lp, history_cap));
+  return 0;
+}
+
+int sovlp_server_create(int slots, const sovlp_params* params, int precision, int block,
+                        int phases_per_launch, void* workspace, int64_t workspace_bytes,
+                        sovlp_stream_t stream, sovlp_server** out) {
+  if (slots < 1 || !params || !workspace || !out) return fail(1, "bad server arguments");
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  if (block != 64 && block != 256) return fail(1, "block must be 64 or 256");
+  if (phases_per_launch < 1 || phases_per_launch > 4096)
+    return fail(1, "phases_per_launch must be in [1, 4096]");
+  if (int rc = check_params(params, precision)) return rc;
+  sovlp_server* h = new sovlp_server{};
+  int rc = precision == 64
+               ? server_create_T<double>(slots, params, block, phases_per_launch, workspace,
+                                         workspace_bytes, S(stream), h)
+               : server_create_T<float>(slots, params, block, phases_per_launch, workspace,
+                                        workspace_bytes, S(stream), h);
+  if (rc) {
+    sovlp_server_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int sovlp_server_admit(sovlp_server* h, int slot, const sovlp_lp* lp, const double* x0,
+                       const double* y0, double omega0, void* workspace,
+                       int64_t workspace_bytes, int64_t history_cap, sovlp_stream_t stream) {
+  if (!h || !lp || !workspace) return fail(1, "null argument");
+  if (slot < 0 || slot >= h->slots) return fail(1, "slot out of range");
+  if (h->busy[slot]) return fail(1, "slot is occupied (release it first)");
+  if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) return fail(1, "negative dimension");
+  if (lp->nnz >= (int64_t)INT32_MAX) return fail(4, "more than 2^31-1 nonzeros");
+  int rc = h->precision == 64
+               ? server_admit_T<double>(h, slot, lp, x0, y0, omega0, workspace, workspace_bytes,
+                                        history_cap, S(stream))
+               : server_admit_T<float>(h, slot, lp, x0, y0, omega0, workspace, workspace_bytes,
+                                       history_cap, S(stream));
+  if (rc) return rc;
+  h->busy[slot] = 1;
+  h->seen_done[slot] = 0;
+  return 0;
+}
+
+int sovlp_server_step(sovlp_server* h, int launches, sovlp_stream_t stream, int32_t* finished,
+                      int32_t* running) {
+  if (!h) return fail(1, "null handle");
+  hipStream_t st = S(stream);
+  for (int i = 0; i < launches; ++i) HIPCHK(hipGraphLaunch(h->exec, st));
+  HIPCHK(hipMemcpyAsync(h->hsnap, h->states, sizeof(LpState) * (size_t)h->slots,
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  int run = 0;
+  for (int i = 0; i < h->slots; ++i) {
+    const bool fin = h->busy[i] && h->hsnap[i].phase == PH_DONE;
+    if (fin) h->seen_done[i] = 1;
+    if (finished) finished[i] = fin ? 1 : 0;
+    run += (h->busy[i] && !fin) ? 1 : 0;
+  }
+  if (running) *running = run;
+  return 0;
+}
+
+int sovlp_server_info(sovlp_server* h, int slot, sovlp_info* info) {
+  if (!h || !info) return fail(1, "null argument");
+  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
+  fill_info(h->hsnap[slot], info);
+  return 0;
+}
+
+int sovlp_server_outputs(sovlp_server* h, int slot, double** x, double** y, double** d) {
+  if (!h) return fail(1, "null handle");
+  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
+  if (x) *x = h->xo[slot];
+  if (y) *y = h->yo[slot];
+  if (d) *d = h->dout[slot];
+  return 0;
+}
+
+int sovlp_server_refinement(sovlp_server* h, int slot, double** rows) {
+  if (!h || !rows) return fail(1, "null argument");
+  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
+  *rows = h->rhist[slot];
+  return 0;
+}
+
+int sovlp_server_phase_counts(sovlp_server* h, int slot, int64_t* counts,
+                              sovlp_stream_t stream) {
+  if (!h || !counts) return fail(1, "null argument");
+  if (slot < 0 || slot >= h->slots) return fail(1, "slot out of range");
+  HIPCHK(hipMemcpyAsync(counts, h->pcount + (size_t)slot * kPhaseSlots,
+                        sizeof(long long) * kPhaseSlots, hipMemcpyDeviceToHost, S(stream)));
+  HIPCHK(hipStreamSynchronize(S(stream)));
+  return 0;
+}
+
+int sovlp_server_release(sovlp_server* h, int slot) {
+  if (!h) return fail(1, "null handle");
+  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
+  if (!h->seen_done[slot]) return fail(1, "the slot's LP has not finished (step until it has)");
+  h->busy[slot] = 0;
+  h->seen_done[slot] = 0;
+  h->xo[slot] = h->yo[slot] = h->dout[slot] = h->rhist[slot] = nullptr;
+  return 0;
+}
+
+int sovlp_server_clock_hz(sovlp_server* h, double* hz) {
+  if (!h || !hz) return fail(1, "null argument");
+  *hz = h->hz;
+  return 0;
+}
+
+int sovlp_server_destroy(sovlp_server* h) {
+  if (!h) return 0;
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
+  if (h->hstage) (void)hipHostFree(h->hstage);
+  if (h->hinit) (void)hipHostFree(h->hinit);
+  if (h->hsnap) (void)hipHostFree(h->hsnap);
+  delete h;
+  return 0;
+}
+
+}  // extern "C"

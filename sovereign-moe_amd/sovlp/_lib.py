@@ -95,6 +95,20 @@ _SIGS = {
     "sovlp_batch_refinement": (C.c_int, [P, C.c_int, C.POINTER(P)]),
     "sovlp_batch_phase_counts": (C.c_int, [P, P, P]),
     "sovlp_batch_destroy": (C.c_int, [P]),
+    "sovlp_server_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.POINTER(I64)]),
+    "sovlp_server_lp_bytes": (C.c_int, [C.POINTER(sovlp_lp), C.c_int, I64, C.POINTER(I64)]),
+    "sovlp_server_create": (C.c_int, [C.c_int, C.POINTER(sovlp_params), C.c_int, C.c_int,
+                                      C.c_int, P, I64, P, C.POINTER(P)]),
+    "sovlp_server_admit": (C.c_int, [P, C.c_int, C.POINTER(sovlp_lp), P, P, F64, P, I64, I64,
+                                     P]),
+    "sovlp_server_step": (C.c_int, [P, C.c_int, P, P, C.POINTER(I32)]),
+    "sovlp_server_info": (C.c_int, [P, C.c_int, C.POINTER(sovlp_info)]),
+    "sovlp_server_outputs": (C.c_int, [P, C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+    "sovlp_server_refinement": (C.c_int, [P, C.c_int, C.POINTER(P)]),
+    "sovlp_server_phase_counts": (C.c_int, [P, C.c_int, P, P]),
+    "sovlp_server_release": (C.c_int, [P, C.c_int]),
+    "sovlp_server_clock_hz": (C.c_int, [P, C.POINTER(F64)]),
+    "sovlp_server_destroy": (C.c_int, [P]),
 }
 
 

@@ -125,13 +125,18 @@ class DeviceLP:
     """A problem's arrays on the device in the layout the engines take, with the
     ``sovlp_lp`` descriptor pointing at them. Keeps the tensors alive."""
 
-    def __init__(self, prob: Problem, precision: int = 64):
+    def __init__(self, prob: Problem, precision: int = 64,
+                 geometry: tuple[int, int] | None = None, v0=None):
+        """``geometry`` = (tile rows, tile nonzeros) of the engine that will run
+        the LP (default: the single-LP engine's); ``v0`` a device tensor whose
+        first ``n`` entries are ``default_rng(0).standard_normal(n)`` (default:
+        a fresh one)."""
         import torch
         flip = prob.sense == ObjSense.MAXIMISE
         A = prob.A
         if A.nnz >= 2**31 - 1:
             raise ValueError("more than 2^31-1 nonzeros")
-        block, tnnz = tile_geometry(precision)
+        block, tnnz = geometry if geometry is not None else tile_geometry(precision)
         self.tiles_a = plan_tiles(A.rp, tnnz, block)
         self.tiles_t = plan_tiles(A.cp, tnnz, block)
         dev = "cuda"
@@ -146,8 +151,12 @@ class DeviceLP:
             cp=up(A.cp, np.int32), ci=up(A.ci, np.int32), cx=up(A.cx, np.float64),
             c=up(c, np.float64), rl=up(prob.row_lb, np.float64), ru=up(prob.row_ub, np.float64),
             lo=up(prob.col_lb, np.float64), hi=up(prob.col_ub, np.float64),
-            v0=up(np.random.default_rng(0).standard_normal(A.n), np.float64),
             tiles_a=up(self.tiles_a, np.int32), tiles_t=up(self.tiles_t, np.int32))
+        if v0 is None:
+            v0 = up(np.random.default_rng(0).standard_normal(A.n), np.float64)
+        elif v0.numel() < A.n:
+            raise ValueError("v0 shorter than the LP's column count")
+        self.t["v0"] = v0
         t = self.t
         self.desc = _lib.sovlp_lp(
             A.m, A.n, A.nnz, t["rp"].data_ptr(), t["ri"].data_ptr(), t["rx"].data_ptr(),

@@ -161,107 +161,82 @@ def uneven_batch(count: int = 4096, base_seed: int = 1000) -> list[KnownOptimum]
 
 def blending_lp(seed: int, K: int | None = None, P: int | None = None,
                 T: int | None = None, *, max_T: int = 720) -> Problem:
-    """A multi-period blending LP.
+    """A multi-period blending LP, feasible by construction.
 
-    Variables: flows ``f[k,p,t] in [0, cap_k]`` (component k into product p in
-    hour t) and inventories ``s[k,t] in [0, INF)``. Rows per hour:
+    Sizes: ``K ~ U{20..60}`` components, ``P ~ U{5..20}`` products, ``T ~
+    U{24..max_T}`` hours (each may be fixed by the caller). Variables: flows
+    ``f[k,p,t] in [0, cap_k]`` (component k into product p in hour t) and end-of-hour
+    inventories ``s[k,t] in [0, INF)``. Rows of hour t:
 
     * inventory balance (equality) per component:
-      ``s[k,t] - s[k,t-1] + Σ_p f[k,p,t] = 0`` ... written with the component's
-      arrival ``a[k,t]`` on the right: ``s[k,t] - s[k,t-1] + Σ_p f = a[k,t]``;
+      ``s[k,t] - s[k,t-1] + Σ_p f[k,p,t] = a[k,t]`` (``a`` = the hour's arrivals;
+      no ``s[k,t-1]`` term at t = 0);
     * supply (``<=``) per component: ``Σ_p f[k,p,t] <= avail_k``;
     * demand (equality) per product: ``Σ_k f[k,p,t] = d_p``;
-    * three quality rows per product: ``Σ_k (q_k - qmax_p) f[k,p,t] <= 0``.
+    * three quality rows per product: ``Σ_k (q[k,j] - qmax[p,j]) f[k,p,t] <= 0``.
 
-    ``qmax_p`` sits 5 % above the quality of a random feasible blend, so every
-    hour is feasible when arrivals cover the demand; arrivals are drawn to do so.
-    The objective minimises component cost minus product revenue.
+    Draws: ``cap ~ U(50,500)``, ``avail ~ U(100,1000)``, ``d ~ U(50,300)``,
+    ``q ~ U(0,1)``, component price ``U(40,80)``, product price ``U(70,110)``.
+    Feasibility: a random blend ``w[p,:]`` (Dirichlet, weighted by the caps) is
+    fixed first; its flows ``d_p w[p,k]`` and per-component use ``u_k`` define
+    ``qmax[p,j] = 1.05 (w[p,:] · q[:,j])``, raise ``cap_k`` and ``avail_k`` to
+    at least 1.05 times what the blend needs (a no-op for most components), and
+    set arrivals ``a[k,t] = u_k U(1.0, 1.3)``, so the blend with its inventories
+    satisfies every row. The objective is component cost minus product revenue.
     """
     rng = np.random.default_rng(seed)
-    K = int(rng.integers(20, 61)) if K is None else K
-    P = int(rng.integers(5, 21)) if P is None else P
-    T = int(rng.integers(24, max_T + 1)) if T is None else T
+    K = int(rng.integers(20, 61)) if K is None else int(K)
+    P = int(rng.integers(5, 21)) if P is None else int(P)
+    T = int(rng.integers(24, max_T + 1)) if T is None else int(T)
     cap = rng.uniform(50.0, 500.0, K)
     avail = rng.uniform(100.0, 1000.0, K)
     demand = rng.uniform(50.0, 300.0, P)
     q = rng.uniform(0.0, 1.0, (K, 3))
     price_k = rng.uniform(40.0, 80.0, K)
     price_p = rng.uniform(70.0, 110.0, P)
-    # qmax: 5% above the quality of a random feasible blend (weights on a random
-    # subset of components), so the blend itself satisfies every quality row
-    qmax = np.empty((P, 3))
-    for p in range(P):
-        w = rng.dirichlet(np.ones(K))
-        qmax[p] = (w @ q) * 1.05
-    # arrivals: enough per hour to cover total demand spread over components
-    need = demand.sum()
-    arrive = rng.uniform(0.5, 1.5, (K, T)) * (need / K) * 1.2
+    w = rng.dirichlet(np.ones(K), P) * cap[None, :]
+    w /= w.sum(axis=1, keepdims=True)                   # (P, K) blend fractions
+    flow = demand[:, None] * w                          # the blend's f[k,p,t], any t
+    use = flow.sum(axis=0)                              # (K,) per-hour use of each component
+    qmax = 1.05 * (w @ q)                               # (P, 3)
+    cap = np.maximum(cap, 1.05 * flow.max(axis=0))
+    avail = np.maximum(avail, 1.05 * use)
+    arrive = use[:, None] * rng.uniform(1.0, 1.3, (K, T))
 
-    nf = K * P * T
-    ns = K * T
+    nf, ns = K * P * T, K * T
     n = nf + ns
-
-    def fidx(k, p, t):
-        return (t * K + k) * P + p
-
-    def sidx(k, t):
-        return nf + t * K + k
-
-    rows, cols, vals = [], [], []
-    rl, ru = [], []
-    r = 0
-    kk, pp = np.meshgrid(np.arange(K), np.arange(P), indexing="ij")
-    kk, pp = kk.ravel(), pp.ravel()
-    for t in range(T):
-        # balance rows (K)
-        for k in range(K):
-            rows += [r] * (P + 1)
-            cols += [fidx(k, p, t) for p in range(P)] + [sidx(k, t)]
-            vals += [1.0] * P + [1.0]
-            if t > 0:
-                rows.append(r)
-                cols.append(sidx(k, t - 1))
-                vals.append(-1.0)
-            rl.append(arrive[k, t])
-            ru.append(arrive[k, t])
-            r += 1
-        # supply rows (K)
-        for k in range(K):
-            rows += [r] * P
-            cols += [fidx(k, p, t) for p in range(P)]
-            vals += [1.0] * P
-            rl.append(-INF)
-            ru.append(avail[k])
-            r += 1
-        # demand rows (P)
-        for p in range(P):
-            rows += [r] * K
-            cols += [fidx(k, p, t) for k in range(K)]
-            vals += [1.0] * K
-            rl.append(demand[p])
-            ru.append(demand[p])
-            r += 1
-        # quality rows (3P)
-        for p in range(P):
-            for j in range(3):
-                coef = q[:, j] - qmax[p, j]
-                rows += [r] * K
-                cols += [fidx(k, p, t) for k in range(K)]
-                vals += coef.tolist()
-                rl.append(-INF)
-                ru.append(0.0)
-                r += 1
-    m = r
-    A = SparseMatrix.from_triplets(np.array(rows), np.array(cols), np.array(vals), m, n)
+    R = 2 * K + 4 * P                                   # rows per hour
+    m = R * T
+    t3, k3, p3 = np.meshgrid(np.arange(T), np.arange(K), np.arange(P), indexing="ij")
+    t3, k3, p3 = t3.ravel(), k3.ravel(), p3.ravel()
+    fcol = (t3 * K + k3) * P + p3
+    base = t3 * R
+    tk_t, tk_k = np.meshgrid(np.arange(T), np.arange(K), indexing="ij")
+    tk_t, tk_k = tk_t.ravel(), tk_k.ravel()
+    scol = nf + tk_t * K + tk_k
+    prev = tk_t > 0
+    qual_r = (base[:, None] + 2 * K + P + 3 * p3[:, None] + np.arange(3)[None, :]).ravel()
+    qual_v = (q[k3] - qmax[p3]).ravel()
+    rows = np.concatenate([base + k3, tk_t * R + tk_k, (tk_t * R + tk_k)[prev],
+                           base + K + k3, base + 2 * K + p3, qual_r])
+    cols = np.concatenate([fcol, scol, (scol - K)[prev], fcol, fcol, np.repeat(fcol, 3)])
+    vals = np.concatenate([np.ones(nf), np.ones(ns), -np.ones(int(prev.sum())), np.ones(nf),
+                           np.ones(nf), qual_v])
+    A = SparseMatrix.from_triplets(rows, cols, vals, m, n)
+    hour_lb = np.concatenate([np.zeros(K), np.full(K, -INF), demand, np.full(3 * P, -INF)])
+    hour_ub = np.concatenate([np.zeros(K), avail, demand, np.zeros(3 * P)])
+    rl = np.tile(hour_lb, T)
+    ru = np.tile(hour_ub, T)
+    bal = (np.arange(T)[:, None] * R + np.arange(K)[None, :]).ravel()
+    rl[bal] = arrive.T.ravel()
+    ru[bal] = arrive.T.ravel()
     c = np.zeros(n)
-    for t in range(T):
-        base = t * K * P
-        c[base:base + K * P] = (price_k[kk] - price_p[pp])
+    c[:nf] = price_k[k3] - price_p[p3]
     col_lb = np.zeros(n)
     col_ub = np.full(n, INF)
-    col_ub[:nf] = np.tile(np.repeat(cap, P), T)
-    return Problem(A=A, c=c, row_lb=np.array(rl), row_ub=np.array(ru), col_lb=col_lb,
-                   col_ub=col_ub, name=f"blend_K{K}_P{P}_T{T}_s{seed}")
+    col_ub[:nf] = cap[k3]
+    return Problem(A=A, c=c, row_lb=rl, row_ub=ru, col_lb=col_lb, col_ub=col_ub,
+                   name=f"blend_K{K}_P{P}_T{T}_s{seed}")
 
 
 def blending_stream(rate: float = 200.0, duration: float = 60.0, seed: int = 0,

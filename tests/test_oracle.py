@@ -148,3 +148,22 @@ def test_planted_mistakes_are_caught():
     q.A = shifted
     r = oracle.solve(q, eps_abs=1e-8, eps_rel=1e-8, max_iter=20000)
     assert r.status != 0 or abs(r.objective - ko.f_star) > 1e-3 * max(1.0, abs(ko.f_star))
+
+
+BLEND = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "blending_highs.json")))
+
+
+@pytest.mark.parametrize("case", BLEND["cases"][:6],
+                         ids=[f"s{c['seed']}_p{c['perturb_seed']}" for c in BLEND["cases"][:6]])
+def test_oracle_reaches_highs_objective_on_blending_lps(case):
+    # the served workload (multi-period blending, and its 1 % resolves) has no
+    # closed form: SciPy's HiGHS objectives pin it (tests/golden/make_blending_vectors.py)
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from golden.make_blending_vectors import build
+    p = build(case)
+    assert (p.m, p.n, p.A.nnz) == (case["m"], case["n"], case["nnz"])
+    r = oracle.solve(p, eps_abs=1e-8, eps_rel=1e-8)
+    assert r.status == 0
+    f = case["objective"]
+    assert abs(r.objective - f) <= 1e-6 * max(1.0, abs(f))
