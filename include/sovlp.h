@@ -102,8 +102,9 @@ int sovlp_plan_tiles(int rows, const int64_t *ptr, int tile_nnz, int tile_rows,
 int sovlp_partition_tiles(int ntiles, const int32_t *tiles, const int64_t *ptr, int parts,
                           int32_t *part_out);
 
-/* Tile geometry compiled into the engines. */
-int sovlp_tile_geometry(int precision, int *block, int *tile_nnz);
+/* Tile geometry compiled into the single-LP engine and the tiled SpMV: the
+ * row cap and nonzero cap of a tile (plan tiles with these). */
+int sovlp_tile_geometry(int precision, int *tile_rows, int *tile_nnz);
 
 /* y = A x through the engines' tile pass (ptr32 = int32 row pointers).
  * `part` has grid+1 entries (sovlp_partition_tiles with parts = grid). */

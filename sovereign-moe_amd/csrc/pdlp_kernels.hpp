@@ -13,6 +13,13 @@
 
 namespace sov {
 
+// LDS of a phase: the products of one tile, two vectors of T (the CHECK
+// phase) or one vector of doubles (scaling phases, FP64 check passes)
+template <class T, int TNNZ>
+constexpr int lds_doubles() {
+  return sizeof(T) == 8 ? 2 * TNNZ : TNNZ;
+}
+
 struct StepCtl {             // device-resident control block (read by every launch)
   Params P;
   long long ticks;           // time budget in device clock ticks, taken when a solve starts
@@ -91,7 +98,7 @@ template <class T, int BLOCK, int TNNZ>
 __global__ void __launch_bounds__(BLOCK)
     pdlp_run_kernel(const LpDev<T>* __restrict__ lps, const StepCtl* __restrict__ ctl,
                     unsigned* done, int max_phases) {
-  __shared__ __align__(16) double lds_raw[2 * TNNZ];
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   __shared__ LpState ss;
   const LpDev<T>& L = lps[blockIdx.x];
@@ -127,7 +134,7 @@ __global__ void __launch_bounds__(BLOCK)
 // W workgroups share one LP's phase; each writes its partial sums.
 template <class T, int BLOCK, int TNNZ>
 __global__ void __launch_bounds__(BLOCK) pdlp_phase_kernel(const LpDev<T>* __restrict__ lp) {
-  __shared__ __align__(16) double lds_raw[2 * TNNZ];
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   const LpDev<T>& L = *lp;
   const LpState s = *L.st;

@@ -106,6 +106,14 @@ struct LpState {             // everything the loop carries between phases
   int conv_last;             // the last X evaluated satisfied _converged
   int absorbed;              // xacc already holds the current round's x
   int ref_rows;              // rows written to rhist
+  // best (X, Y) of the refinement so far, by the _converged ratio
+  // max(pres / ptol, dres / dtol, gap / gtol) (<= 1: converged)
+  double best_q;
+  double best_fin[5];        // (pres, dres, gap, pobj, dobj) of the best
+  int nobetter;              // consecutive evaluations that did not improve on it
+  int save_best;             // the next REF_SHIFT copies X, Y to xbest, ybest
+  int use_best;              // report xbest, ybest rather than X, Y
+  int pad_best;
 };
 
 template <class T>
@@ -142,6 +150,7 @@ struct LpDev {
   // q is the slack cost in scaled units (zero in round 0)
   double *yacc, *dacc, *cc, *qc;
   T* q;
+  double *xbest, *ybest;     // the refinement's best X, Y (FP32 engines)
   double* rhist;             // [kMaxRounds][5] per round: k, pres, dres, gap, pobj
   LpState* st;
   long long* pcount;         // [kPhaseSlots] phase counters
@@ -598,7 +607,8 @@ struct PhaseWork {
         for (long long j = j0 + tid; j < j1; j += BLOCK) {
           double v = (double)l.x[j] * l.colf[j];
           if (l.xacc) {     // X plus the current round's correction, if not yet in X
-            v = l.xacc[j] + (s.absorbed ? 0.0 : fmin(fmax(v, l.clo[j]), l.chi[j]) / rs);
+            v = s.use_best ? l.xbest[j]
+                           : l.xacc[j] + (s.absorbed ? 0.0 : fmin(fmax(v, l.clo[j]), l.chi[j]) / rs);
           }
           v = fmin(fmax(v, l.lo0[j]), l.hi0[j]);
           l.xo[j] = v;
@@ -606,7 +616,7 @@ struct PhaseWork {
         }
         for (long long i = i0 + tid; i < i1; i += BLOCK) {
           double v = ((double)l.y[i] * l.rowf[i]) / s.sobj;
-          if (l.xacc) v = l.yacc[i] + (s.absorbed ? 0.0 : v / rs);
+          if (l.xacc) v = s.use_best ? l.ybest[i] : l.yacc[i] + (s.absorbed ? 0.0 : v / rs);
           l.yo[i] = l.flip ? v : -v;
         }
       } break;
@@ -660,8 +670,10 @@ struct PhaseWork {
 
       case PH_REF_SHIFT: {  // next round: bounds shifted by X (and A X), scaled by Delta
         const double rs = s.ref_scale;
+        const bool save = s.save_best;   // X, Y just evaluated are the best so far
         for (long long j = j0 + tid; j < j1; j += BLOCK) {
           const double X = l.xacc[j], lo0 = l.lo0[j], hi0 = l.hi0[j], cf = l.colf[j];
+          if (save) l.xbest[j] = X;
           const double lo = lo0 > -kInf ? fmax((lo0 - X) * rs, -kTrustRegion) : -kInf;
           const double hi = hi0 < kInf ? fmin((hi0 - X) * rs, kTrustRegion) : kInf;
           l.wlo[j] = lo;
@@ -692,6 +704,7 @@ struct PhaseWork {
           l.ru[i] = (T)(ru < kInf ? ru * rf : kInf);
           if (rl > -kInf) part[0] = nanmax(part[0], fabs(rl));
           if (ru < kInf) part[0] = nanmax(part[0], fabs(ru));
+          if (save) l.ybest[i] = l.yacc[i];
           // row-slack cost -Delta Y (unscaled and scaled)
           const double qv = -(l.yacc[i] * rs);
           l.qc[i] = qv;
@@ -792,26 +805,35 @@ struct PhaseWork {
     tile_pass<BLOCK, TNNZ, 2, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prodT, epiT, lds);
 
     if constexpr (!kDerived) {
-      auto prodA0 = [&](int q, int j, double (&p)[2]) {
-        const double a = l.av[q], cf = l.colf[j];
-        p[0] = a * ((double)l.x[j] * cf);
-        p[1] = a * ((double)(T)((double)l.sumx[j] / inv_se) * cf);
+      // FP64 products of the original matrix, one vector per pass so that the
+      // LDS holds TNNZ doubles: the current point's sums wait in tmpm / tmpn
+      // (the same lane handles a row in both passes: same tile plan)
+      auto prodA0 = [&](int q, int j, double (&p)[1]) {
+        p[0] = l.av[q] * ((double)l.x[j] * l.colf[j]);
       };
-      auto epiA0 = [&](int i, const double (&sm)[2]) {
+      auto epiA0 = [&](int i, const double (&sm)[1]) { l.tmpm[i] = sm[0]; };
+      tile_pass<BLOCK, TNNZ, 1, false, double>(l.ap, l.ai, l.tilesA, a0, a1, prodA0, epiA0, ldsd);
+      auto prodA1 = [&](int q, int j, double (&p)[1]) {
+        p[0] = l.av[q] * ((double)(T)((double)l.sumx[j] / inv_se) * l.colf[j]);
+      };
+      auto epiA1 = [&](int i, const double (&sm)[1]) {
         const double y = l.y[i], ya = (T)((double)l.sumy[i] / inv_se);
-        unscaled_row(i, sm[0], sm[1], y, ya);
+        unscaled_row(i, l.tmpm[i], sm[0], y, ya);
       };
-      tile_pass<BLOCK, TNNZ, 2, false, double>(l.ap, l.ai, l.tilesA, a0, a1, prodA0, epiA0, ldsd);
-      auto prodT0 = [&](int q, int i, double (&p)[2]) {
-        const double a = l.tv[q], rf = l.rowf[i];
-        p[0] = a * (((double)l.y[i] * rf) / so);
-        p[1] = a * (((double)(T)((double)l.sumy[i] / inv_se) * rf) / so);
+      tile_pass<BLOCK, TNNZ, 1, false, double>(l.ap, l.ai, l.tilesA, a0, a1, prodA1, epiA1, ldsd);
+      auto prodT0 = [&](int q, int i, double (&p)[1]) {
+        p[0] = l.tv[q] * (((double)l.y[i] * l.rowf[i]) / so);
       };
-      auto epiT0 = [&](int j, const double (&sm)[2]) {
+      auto epiT0 = [&](int j, const double (&sm)[1]) { l.tmpn[j] = sm[0]; };
+      tile_pass<BLOCK, TNNZ, 1, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prodT0, epiT0, ldsd);
+      auto prodT1 = [&](int q, int i, double (&p)[1]) {
+        p[0] = l.tv[q] * (((double)(T)((double)l.sumy[i] / inv_se) * l.rowf[i]) / so);
+      };
+      auto epiT1 = [&](int j, const double (&sm)[1]) {
         const double x = l.x[j], xa = (T)((double)l.sumx[j] / inv_se);
-        unscaled_col(j, sm[0], sm[1], x, xa);
+        unscaled_col(j, l.tmpn[j], sm[0], x, xa);
       };
-      tile_pass<BLOCK, TNNZ, 2, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prodT0, epiT0, ldsd);
+      tile_pass<BLOCK, TNNZ, 1, false, double>(l.tp, l.ti, l.tilesT, b0, b1, prodT1, epiT1, ldsd);
     }
     v[TFLAG] = clock_flag();
   }
@@ -907,6 +929,10 @@ struct Transition {
         s.round = 0;
         s.absorbed = 0;
         s.conv_last = 0;
+        s.best_q = __builtin_inf();
+        s.nobetter = 0;
+        s.save_best = 0;
+        s.use_best = 0;
         s.k_done = 0;
         s.kmax = P.max_iter;
         if (refining()) {
@@ -1033,8 +1059,11 @@ struct Transition {
             s.k_best = s.k;
           }
           if (ratio > 1.0 && (s.k - s.k_best >= 4096 || s.k >= 32768)) {
-            if (s.round > 0 && s.conv_last) {
+            if (s.round > 0 && s.best_q <= 1.0) {
+              // the best X evaluated so far met _converged: report it
               s.absorbed = 1;
+              s.use_best = s.nobetter > 0;
+              for (int k = 0; k < 5; ++k) s.fin[k] = s.best_fin[k];
               s.status = ST_OPTIMAL;
               return PH_FINAL;
             }
@@ -1092,8 +1121,23 @@ struct Transition {
         if (s.round == 0) s.k_round0 = s.k;
         s.k_done += s.k;
         s.k = 0;
-        const bool conv = pres <= s.ptol0 && dres <= s.dtol0 &&
-                          gap <= P.eps_abs + P.eps_rel * (fabs(pou) + fabs(dou));
+        const double gtol = P.eps_abs + P.eps_rel * (fabs(pou) + fabs(dou));
+        const bool conv = pres <= s.ptol0 && dres <= s.dtol0 && gap <= gtol;
+        // the _converged ratio of X, Y (<= 1: converged); the refinement keeps
+        // the best X, Y by it and stops after two evaluations that do not
+        // improve on the best (FP32 rounds can move X the wrong way)
+        const double q = fmax(fmax(pres / fmax(s.ptol0, 1e-300), dres / fmax(s.dtol0, 1e-300)),
+                              gap / fmax(gtol, 1e-300));
+        const bool improved = q < s.best_q;
+        if (improved) {
+          s.best_q = q;
+          s.best_fin[0] = pres; s.best_fin[1] = dres; s.best_fin[2] = gap;
+          s.best_fin[3] = pou; s.best_fin[4] = dou;
+          s.nobetter = 0;
+        } else {
+          ++s.nobetter;
+        }
+        s.save_best = 0;
         const double gap_before = s.gap_last;
         s.conv_last = conv;
         s.gap_last = gap;
@@ -1104,22 +1148,27 @@ struct Transition {
         // refine_tol relative, so the objective is that close to the optimum
         const bool target = pres <= P.refine_tol * (1.0 + s.rhs0) &&
                             gap <= P.refine_tol * (1.0 + fabs(pou) + fabs(dou));
-        if (conv && (target || s.round + 1 >= P.refine_rounds)) {
+        if (conv && target) {
           s.status = ST_OPTIMAL;
           return PH_FINAL;
         }
-        if (s.k_done >= P.max_iter) {
-          s.status = ST_ITERATION_LIMIT;
+        int stop = ST_RUNNING;
+        if (s.k_done >= P.max_iter) stop = ST_ITERATION_LIMIT;
+        else if (s.time_flag) stop = ST_TIME_LIMIT;
+        else if (s.round + 1 >= P.refine_rounds || s.nobetter >= 2)
+          stop = ST_NUMERICAL;       // the rounds could not reach the tolerance
+        if (stop != ST_RUNNING) {
+          // report the best X, Y (the current ones if they are the best);
+          // OPTIMAL whenever the best met _converged
+          s.use_best = !improved;
+          if (s.best_q <= 1.0) {
+            stop = ST_OPTIMAL;
+            for (int k = 0; k < 5; ++k) s.fin[k] = s.best_fin[k];
+          }
+          s.status = stop;
           return PH_FINAL;
         }
-        if (s.time_flag) {
-          s.status = ST_TIME_LIMIT;
-          return PH_FINAL;
-        }
-        if (s.round + 1 >= P.refine_rounds) {
-          s.status = ST_NUMERICAL;   // the rounds could not reach the tolerance
-          return PH_FINAL;
-        }
+        s.save_best = improved;
         // Delta: the power of two nearest below 1 / error, at most 2^20 times
         // the last one (and never below 1).  The error is the larger of the
         // residuals and the gap over the cost scale, which estimates how far X

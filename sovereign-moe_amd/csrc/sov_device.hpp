@@ -118,6 +118,10 @@ __device__ __forceinline__ void block_combine(T (&s)[NV], T* scratch) {
   __syncthreads();
 }
 
+// Rows per lane of the short-row tiles: a tile holds at most kRowsPerLane *
+// BLOCK rows (the host plans tiles with that row cap).
+constexpr int kRowsPerLane = 4;
+
 // ------------------------------------------------------------------------
 // CSR-Adaptive tile pass.
 //   ptr   : row pointers (LP-local int32 offsets), rows+1 entries
@@ -164,6 +168,58 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
       if (more) np1 = uniform(ptr[nr1]);
       block_combine<BLOCK, NV, MAXOP, T>(s, lds);
       if (threadIdx.x == 0) epi(r0, s, pr);
+      __syncthreads();
+    } else if (rows > BLOCK) {
+      // many short rows: one lane per row, up to kRowsPerLane rows per lane,
+      // every row's extent and epilogue operands loaded before the products
+      int a[kRowsPerLane], b[kRowsPerLane];
+      decltype(pre(0)) pr[kRowsPerLane];
+#pragma unroll
+      for (int k = 0; k < kRowsPerLane; ++k) {
+        const int row = (int)threadIdx.x + k * BLOCK;
+        a[k] = b[k] = 0;
+        pr[k] = decltype(pre(0)){};
+        if (row < rows) {
+          a[k] = ptr[r0 + row] - p0;
+          b[k] = ptr[r0 + row + 1] - p0;
+          pr[k] = pre(r0 + row);
+        }
+      }
+      for (int q = (int)threadIdx.x; q < nz; q += BLOCK) {
+        T v[NV];
+        prod(p0 + q, idx[p0 + q], v);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[k * TNNZ + q] = v[k];
+      }
+      __syncthreads();
+      if (more) np1 = uniform(ptr[nr1]);
+#pragma unroll
+      for (int k = 0; k < kRowsPerLane; ++k) {
+        const int row = (int)threadIdx.x + k * BLOCK;
+        if (row < rows) {
+          T s[NV];
+#pragma unroll
+          for (int u = 0; u < NV; ++u) s[u] = T(0);
+          for (int q = a[k]; q < b[k]; ++q) {
+#pragma unroll
+            for (int u = 0; u < NV; ++u) s[u] = MAXOP ? tmax(s[u], lds[u * TNNZ + q]) : s[u] + lds[u * TNNZ + q];
+          }
+          epi(r0 + row, s, pr[k]);
+        }
+      }
+      // rows past kRowsPerLane * BLOCK (a plan with a larger row cap): no prefetch
+      for (int row = (int)threadIdx.x + kRowsPerLane * BLOCK; row < rows; row += BLOCK) {
+        const int ra = ptr[r0 + row] - p0, rb = ptr[r0 + row + 1] - p0;
+        const auto pq = pre(r0 + row);
+        T s[NV];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) s[u] = T(0);
+        for (int q = ra; q < rb; ++q) {
+#pragma unroll
+          for (int u = 0; u < NV; ++u) s[u] = MAXOP ? tmax(s[u], lds[u * TNNZ + q]) : s[u] + lds[u * TNNZ + q];
+        }
+        epi(r0 + row, s, pq);
+      }
       __syncthreads();
     } else {
       int tpr = BLOCK / (rows > 0 ? rows : 1);

@@ -41,8 +41,8 @@ int fail(int code, const std::string& msg) {
 inline hipStream_t S(sovlp_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
 constexpr int kBlock = 256;
-constexpr int kTnnz64 = 2048;   // FP64 engines: 2 x 2048 x 8 B = 32 KiB LDS
-constexpr int kTnnz32 = 2048;   // FP32 engines: same tiles (the FP64 check passes share the LDS)
+constexpr int kTnnz64 = 2048;   // FP64 engines: 2 vectors x 2048 x 8 B = 32 KiB LDS
+constexpr int kTnnz32 = 4096;   // FP32 engines: 2 x 4096 x 4 B (or 4096 doubles) = 32 KiB
 
 // ---------------------------------------------------------------- Backend
 template <class T, int TPR>
@@ -222,6 +222,8 @@ void carve_lp(Carver& cv, LpDev<T>& L, int m, int n, long long nnz, long long hi
     L.dacc = cv.take<double>(n);
     L.cc = cv.take<double>(n);
     L.q = reinterpret_cast<T*>(cv.take<float>(m));
+    L.xbest = cv.take<double>(n);
+    L.ybest = cv.take<double>(m);
   }
 }
 
@@ -266,9 +268,9 @@ int sovlp_clock_rate(int device, double* hz) {
   return 0;
 }
 
-int sovlp_tile_geometry(int precision, int* block, int* tile_nnz) {
+int sovlp_tile_geometry(int precision, int* tile_rows, int* tile_nnz) {
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  *block = kBlock;
+  *tile_rows = kRowsPerLane * kBlock;
   *tile_nnz = precision == 64 ? kTnnz64 : kTnnz32;
   return 0;
 }
@@ -911,7 +913,7 @@ int sovlp_pdlp_destroy(sovlp_pdlp* h) {
 int sovlp_batch_geometry(int precision, int block, int* tile_rows, int* tile_nnz) {
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
   if (block != 64 && block != 256) return fail(1, "block must be 64 or 256");
-  *tile_rows = block;
+  *tile_rows = kRowsPerLane * block;
   *tile_nnz = block == 64 ? kTnnzSmall : (precision == 64 ? kTnnz64 : kTnnz32);
   return 0;
 }

@@ -142,6 +142,20 @@ class BatchEngine:
         _lib.call("sovlp_batch_infos", self.h, infos, self.stream)
         return infos
 
+    def outputs_of(self, index: int, infos=None) -> tuple[np.ndarray, np.ndarray, float]:
+        """(x, y, primal weight) of LP ``index`` as reported (original units,
+        textbook signs): the warm start of a re-solve. ``infos``: a recent
+        ``infos()`` result, to skip reading the states again."""
+        import torch
+        x, y, d = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _lib.call("sovlp_batch_outputs", self.h, int(index), C.byref(x), C.byref(y), C.byref(d))
+        m, n = self.db.meta[index][0], self.db.meta[index][1]
+        base = self.ws.data_ptr()
+        xs = self.ws[x.value - base: x.value - base + 8 * n].view(torch.float64)
+        ys = self.ws[y.value - base: y.value - base + 8 * m].view(torch.float64)
+        infos = self.infos() if infos is None else infos
+        return xs.cpu().numpy().copy(), ys.cpu().numpy().copy(), float(infos[index].primal_weight)
+
     def phase_counts(self) -> np.ndarray:
         slots = C.c_int()
         _lib.call("sovlp_phase_slots", C.byref(slots))

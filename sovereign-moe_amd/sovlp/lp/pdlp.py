@@ -105,9 +105,10 @@ _TEAMS = {"auto": 0, "block": 1, "grid": 2}
 
 
 def tile_geometry(precision: int) -> tuple[int, int]:
-    b, t = C.c_int(), C.c_int()
-    _lib.call("sovlp_tile_geometry", int(precision), C.byref(b), C.byref(t))
-    return b.value, t.value
+    """(tile row cap, tile nonzero cap) of the single-LP engine."""
+    r, t = C.c_int(), C.c_int()
+    _lib.call("sovlp_tile_geometry", int(precision), C.byref(r), C.byref(t))
+    return r.value, t.value
 
 
 def plan_tiles(ptr: np.ndarray, tile_nnz: int, tile_rows: int) -> np.ndarray:

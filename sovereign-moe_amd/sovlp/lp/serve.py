@@ -42,7 +42,7 @@ from .batch import batch_geometry
 from .pdlp import DeviceLP, PDLPInfo, PDLPParams, refinement_rows
 
 __all__ = ["Request", "Served", "HbmAdmission", "hbm_budget", "DeviceServer", "ServeLoop",
-           "summarize", "WarmStart"]
+           "summarize", "WarmStart", "resolve_warm"]
 
 
 @dataclass
@@ -393,3 +393,17 @@ def summarize(served: list[Served], span: float | None = None) -> dict:
         if warm else float("nan"),
         "span_s": span,
     }
+
+
+def resolve_warm(device, problems: list[Problem], warm: list[WarmStart],
+                 budget: int = 1 << 40) -> list[Solution]:
+    """Solve ``problems`` on ``device`` (normally an FP64 ``DeviceServer``), each
+    warm-started from ``warm[i]``: the escalation of LPs that a batch solved in
+    FP32 ended NUMERICAL or at the iteration limit."""
+    loop = ServeLoop(device, HbmAdmission(budget))
+    reqs = []
+    for i, (p, w) in enumerate(zip(problems, warm)):
+        loop.warm[i] = w
+        reqs.append(Request(i, p, 0.0, model=i, resolve=True))
+    out = {s.key: s for s in loop.run(reqs)}
+    return [out[i].solution for i in range(len(problems))]

@@ -89,9 +89,12 @@ def phase_bytes(name: str, m: int, n: int, nnz: int, ntiles_a: int, ntiles_t: in
             # unscaled KKT from the same products: row/col factors and original bounds
             v += 3 * _F64 * m + 4 * _F64 * n
         else:
-            # unscaled KKT recomputed in FP64 from the original matrix
-            v += pa + nnz * _F64 + n * (2 * s + _F64) + m * (2 * s + 3 * _F64)
-            v += pt + nnz * _F64 + m * (2 * s + _F64) + n * (2 * s + 4 * _F64)
+            # unscaled KKT recomputed in FP64 from the original matrix, one
+            # vector per pass: current point (sum kept in tmpm / tmpn), then average
+            v += pa + nnz * _F64 + n * (s + _F64) + m * _F64
+            v += pa + nnz * _F64 + n * (s + _F64) + m * (2 * s + 4 * _F64)
+            v += pt + nnz * _F64 + m * (s + _F64) + n * _F64
+            v += pt + nnz * _F64 + m * (s + _F64) + n * (2 * s + 5 * _F64)
         return int(v)
     return int(b.get(name, 0))
 
