@@ -63,6 +63,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="wall budget of the CPU baseline sample (0: skip)")
     ap.add_argument("--spmv", action="store_true", help="also time the SpMV of the 200k x 1M LP")
+    ap.add_argument("--tuning", default="auto",
+                    help="autotuner table (JSON) choosing each LP's kernel configuration; "
+                         "'auto': the packaged MI355X table for the precision if present; "
+                         "'none': every LP on --block")
     ap.add_argument("--escalate", action="store_true",
                     help="FP32: re-solve the LPs the refinement cannot finish in FP64, "
                          "warm-started from their FP32 point, inside the timed step")
@@ -163,8 +167,18 @@ def main():
     log(f"generated {a.lps} LPs in {time.perf_counter() - t:.1f} s "
         f"(sum nnz {sum(p.A.nnz for p in problems)})")
     par = PDLPParams(eps_abs=a.tol, eps_rel=a.tol, feas_cap=a.tol, device="gpu")
+    tuner = None
+    if a.tuning != "none":
+        from sovlp.lp.autotune import Autotuner, packaged_table
+        path = packaged_table(a.precision) if a.tuning == "auto" else a.tuning
+        if path and os.path.exists(path):
+            tuner = Autotuner.load(path)
+            if tuner.precision != a.precision:
+                tuner = None
+            else:
+                log(f"kernel configurations from {os.path.relpath(path, ROOT)}: {tuner.table}")
     t = time.perf_counter()
-    eng = BatchEngine(problems, par, precision=a.precision, block=a.block)
+    eng = BatchEngine(problems, par, precision=a.precision, block=a.block, tuner=tuner)
     log(f"batch laid out on the device in {time.perf_counter() - t:.1f} s")
     fb = None
     if a.precision == 32 and a.escalate:
@@ -224,8 +238,9 @@ def main():
     launch_ms = dev_ms / launches
     achieved = step_bytes / graphs[-1] / (launch_ms * 1e-3) / 1e9
     from sovlp.lp.batch import batch_geometry
-    kernel = (f"pdlp_run_kernel<{'double' if a.precision == 64 else 'float'}, {a.block}, "
-              f"{batch_geometry(a.precision, a.block)[1]}>")
+    blocks = sorted(set(int(b) for b in eng.db.blocks))
+    kernel = " + ".join(f"pdlp_run_kernel<{'double' if a.precision == 64 else 'float'}, {b}, "
+                        f"{batch_geometry(a.precision, b)[1]}>" for b in blocks)
     eng.close()
     if fb is not None:
         fb.close()
@@ -247,7 +262,8 @@ def main():
                                f"(60% eq), n=U[2,4]m, 3-8 nnz/col, bounds [0,U(5,50)], "
                                f"seeds {a.seed}..{a.seed + a.lps - 1}",
                    "lps": a.lps, "tolerance": a.tol, "precision": a.precision,
-                   "block": a.block, "parallelism": "one workgroup per LP"},
+                   "block": a.block, "parallelism": "one workgroup per LP",
+                   "kernel_configs": "autotuned" if tuner is not None else f"block {a.block}"},
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "optimal": opt,

@@ -171,6 +171,7 @@ typedef struct {
   int32_t refine_rows;      /* rows of the refinement record */
   int32_t refine_round;     /* last refinement round started (0: none) */
   double refine_scale;      /* Delta of that round */
+  int64_t start_tick;       /* device clock when the solve started */
 } sovlp_info;
 
 /* Workspace bytes for one LP at `precision` 64 (FP64 iterates) or 32 (FP32
@@ -238,8 +239,8 @@ int sovlp_pdlp_phase_counts(sovlp_pdlp *h, int64_t *counts, sovlp_stream_t strea
  * Tile plans must use sovlp_batch_geometry's tile size and row count. */
 typedef struct sovlp_batch sovlp_batch;
 
-/* Tile geometry (workgroup size and tile nonzeros) of the batched engine
- * for `precision` and `block` (64 or 256 threads per LP). */
+/* Tile geometry of the batched engine for `precision` and `block` (64, 128
+ * or 256 threads per LP): the row cap and nonzero cap of a tile. */
 int sovlp_batch_geometry(int precision, int block, int *tile_rows, int *tile_nnz);
 
 /* Workspace for `count` LPs (host array of descriptors with device pointers). */
@@ -248,11 +249,20 @@ int sovlp_batch_workspace_bytes(int count, const sovlp_lp *lps, int precision,
 
 /* Lays out the batch in the workspace and records one replay into a
  * hipGraph: one launch, one workgroup of `block` threads per LP, each running
- * up to 128 phases of its LP back to back.  `block` is 64 or 256.
+ * up to 128 phases of its LP back to back.  `block` is 64, 128 or 256.
  * Synchronises. */
 int sovlp_batch_create(int count, const sovlp_lp *lps, const sovlp_params *params,
                        int precision, int block, void *workspace, int64_t workspace_bytes,
                        int64_t history_cap, sovlp_stream_t stream, sovlp_batch **out);
+
+/* As sovlp_batch_create with a workgroup size per LP (blocks[count], each 64,
+ * 128 or 256; an autotuner's choice from the LP's own shape).  A replay
+ * launches the LPs of each size in one launch of their own, one after the
+ * other; each LP's tile plans use sovlp_batch_geometry of its own size. */
+int sovlp_batch_create_mixed(int count, const sovlp_lp *lps, const sovlp_params *params,
+                             int precision, const int32_t *blocks, void *workspace,
+                             int64_t workspace_bytes, int64_t history_cap, sovlp_stream_t stream,
+                             sovlp_batch **out);
 
 /* Solves every LP of the batch from scratch: resets the states, then replays
  * the graph until every LP has terminated (or `max_graphs` > 0 replays).
@@ -297,7 +307,7 @@ int sovlp_server_lp_bytes(const sovlp_lp *lp, int precision, int64_t history_cap
                           int64_t *bytes);
 
 /* Creates an empty server (every slot free) and records its launch graph.
- * `block` 64 or 256; params->time_limit is each LP's device time budget,
+ * `block` 64, 128 or 256; params->time_limit is each LP's device time budget,
  * counted from its first phase.  Synchronises. */
 int sovlp_server_create(int slots, const sovlp_params *params, int precision, int block,
                         int phases_per_launch, void *workspace, int64_t workspace_bytes,

@@ -503,6 +503,28 @@ namespace {
 
 constexpr int kTnnzSmall = 512;   // 64-thread workgroups: 2 x 512 x 8 B = 8 KiB LDS
 constexpr int kPairsPerGraph = 64;     // W > 1: phase + decide launches per graph
+
+// Workgroup sizes of the batched engine, and their tile nonzero caps (16 KiB
+// of LDS for 128 threads, 32 KiB for 256)
+bool batch_block_ok(int block) { return block == 64 || block == 128 || block == 256; }
+int batch_tnnz(int precision, int block) {
+  if (block == 64) return kTnnzSmall;
+  if (block == 128) return precision == 64 ? kTnnz64 / 2 : kTnnz32 / 2;
+  return precision == 64 ? kTnnz64 : kTnnz32;
+}
+
+// One launch of the run kernel over `count` LP descriptors with `block` threads each
+template <class T>
+void launch_run(int block, const LpDev<T>* lps, int count, const StepCtl* ctl, unsigned* done,
+                int phases, hipStream_t st) {
+  constexpr int T256 = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
+  if (block == 64)
+    pdlp_run_kernel<T, 64, kTnnzSmall><<<count, 64, 0, st>>>(lps, ctl, done, phases);
+  else if (block == 128)
+    pdlp_run_kernel<T, 128, T256 / 2><<<count, 128, 0, st>>>(lps, ctl, done, phases);
+  else
+    pdlp_run_kernel<T, 256, T256><<<count, 256, 0, st>>>(lps, ctl, done, phases);
+}
 constexpr int kPhasesPerLaunch = 128;  // run kernel: phases per LP per launch
 
 template <class T>
@@ -657,9 +679,21 @@ size_t batch_ws_T(int count, const sovlp_lp* lps, long long hist_cap) {
 }
 
 template <class T>
-int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, int block, void* ws,
-                   int64_t ws_bytes, int64_t hist_cap, hipStream_t st, sovlp_batch* h) {
+int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, const int32_t* blocks,
+                   void* ws, int64_t ws_bytes, int64_t hist_cap, hipStream_t st, sovlp_batch* h) {
   if ((size_t)ws_bytes < batch_ws_T<T>(count, lps, hist_cap)) return fail(1, "workspace too small");
+  // descriptors grouped by workgroup size (one launch per group); pos[i] is
+  // LP i's place in the device array
+  const int kBlocks[3] = {64, 128, 256};
+  std::vector<int> order, pos(count), gcount(3, 0);
+  for (int g = 0; g < 3; ++g)
+    for (int i = 0; i < count; ++i)
+      if (blocks[i] == kBlocks[g]) {
+        pos[i] = (int)order.size();
+        order.push_back(i);
+        ++gcount[g];
+      }
+  if ((int)order.size() != count) return fail(1, "every LP's block must be 64, 128 or 256");
   Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
   LpDev<T>* d_lps = cv.take<LpDev<T>>(count);
   StepCtl* ctl = cv.take<StepCtl>(1);
@@ -673,7 +707,7 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, int 
   h->st.resize(count);
   for (int i = 0; i < count; ++i) {
     if (lps[i].nnz >= (int64_t)INT32_MAX) return fail(4, "an LP has more than 2^31-1 nonzeros");
-    LpDev<T>& L = host[i];
+    LpDev<T>& L = host[pos[i]];
     L = LpDev<T>{};
     carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
     fill_lp<T>(L, lps[i]);
@@ -689,7 +723,7 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, int 
   HIPCHK(hipStreamSynchronize(st));
   h->count = count;
   h->precision = sizeof(T) == 8 ? 64 : 32;
-  h->block = block;
+  h->block = blocks[0];
   h->lps = d_lps;
   h->ctl = ctl;
   h->done = done;
@@ -697,12 +731,9 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, int 
   h->ticks_set = c0.ticks;
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-  if (block == 64)
-    pdlp_run_kernel<T, 64, kTnnzSmall><<<count, 64, 0, h->cap>>>(d_lps, ctl, done,
-                                                                  kPhasesPerLaunch);
-  else
-    pdlp_run_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>
-        <<<count, kBlock, 0, h->cap>>>(d_lps, ctl, done, kPhasesPerLaunch);
+  for (int g = 0, off = 0; g < 3; off += gcount[g], ++g)
+    if (gcount[g] > 0)
+      launch_run<T>(kBlocks[g], d_lps + off, gcount[g], ctl, done, kPhasesPerLaunch, h->cap);
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
   HIPCHK(hipHostMalloc((void**)&h->hdone, 2 * sizeof(unsigned), hipHostMallocDefault));
@@ -747,6 +778,7 @@ void fill_info(const LpState& t, sovlp_info* f) {
   f->refine_rows = t.ref_rows;
   f->refine_round = t.round;
   f->refine_scale = t.ref_scale;
+  f->start_tick = t.tick0;
 }
 
 int check_params(const sovlp_params* p, int precision) {
@@ -912,9 +944,9 @@ int sovlp_pdlp_destroy(sovlp_pdlp* h) {
 
 int sovlp_batch_geometry(int precision, int block, int* tile_rows, int* tile_nnz) {
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  if (block != 64 && block != 256) return fail(1, "block must be 64 or 256");
+  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
   *tile_rows = kRowsPerLane * block;
-  *tile_nnz = block == 64 ? kTnnzSmall : (precision == 64 ? kTnnz64 : kTnnz32);
+  *tile_nnz = batch_tnnz(precision, block);
   return 0;
 }
 
@@ -927,18 +959,21 @@ int sovlp_batch_workspace_bytes(int count, const sovlp_lp* lps, int precision,
   return 0;
 }
 
-int sovlp_batch_create(int count, const sovlp_lp* lps, const sovlp_params* params,
-                       int precision, int block, void* workspace, int64_t workspace_bytes,
-                       int64_t history_cap, sovlp_stream_t stream, sovlp_batch** out) {
-  if (count < 1 || !lps || !params || !workspace || !out) return fail(1, "bad batch arguments");
+int sovlp_batch_create_mixed(int count, const sovlp_lp* lps, const sovlp_params* params,
+                             int precision, const int32_t* blocks, void* workspace,
+                             int64_t workspace_bytes, int64_t history_cap, sovlp_stream_t stream,
+                             sovlp_batch** out) {
+  if (count < 1 || !lps || !params || !blocks || !workspace || !out)
+    return fail(1, "bad batch arguments");
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  if (block != 64 && block != 256) return fail(1, "block must be 64 or 256");
+  for (int i = 0; i < count; ++i)
+    if (!batch_block_ok(blocks[i])) return fail(1, "block must be 64, 128 or 256");
   if (int rc = check_params(params, precision)) return rc;
   sovlp_batch* h = new sovlp_batch{};
   int rc = precision == 64
-               ? batch_create_T<double>(count, lps, params, block, workspace, workspace_bytes,
+               ? batch_create_T<double>(count, lps, params, blocks, workspace, workspace_bytes,
                                         history_cap, S(stream), h)
-               : batch_create_T<float>(count, lps, params, block, workspace, workspace_bytes,
+               : batch_create_T<float>(count, lps, params, blocks, workspace, workspace_bytes,
                                        history_cap, S(stream), h);
   if (rc) {
     sovlp_batch_destroy(h);
@@ -946,6 +981,16 @@ int sovlp_batch_create(int count, const sovlp_lp* lps, const sovlp_params* param
   }
   *out = h;
   return 0;
+}
+
+int sovlp_batch_create(int count, const sovlp_lp* lps, const sovlp_params* params,
+                       int precision, int block, void* workspace, int64_t workspace_bytes,
+                       int64_t history_cap, sovlp_stream_t stream, sovlp_batch** out) {
+  if (count < 1) return fail(1, "bad batch arguments");
+  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
+  std::vector<int32_t> blocks(count, block);
+  return sovlp_batch_create_mixed(count, lps, params, precision, blocks.data(), workspace,
+                                  workspace_bytes, history_cap, stream, out);
 }
 
 int sovlp_batch_run(sovlp_batch* h, double seconds_per_lp, int64_t max_graphs,
@@ -1132,11 +1177,7 @@ int server_create_T(int slots, const sovlp_params* prm, int block, int phases, v
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-  if (block == 64)
-    pdlp_run_kernel<T, 64, kTnnzSmall><<<slots, 64, 0, h->cap>>>(d_lps, ctl, done, phases);
-  else
-    pdlp_run_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>
-        <<<slots, kBlock, 0, h->cap>>>(d_lps, ctl, done, phases);
+  launch_run<T>(block, d_lps, slots, ctl, done, phases, h->cap);
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
   return 0;
@@ -1198,7 +1239,7 @@ int sovlp_server_create(int slots, const sovlp_params* params, int precision, in
                         sovlp_stream_t stream, sovlp_server** out) {
   if (slots < 1 || !params || !workspace || !out) return fail(1, "bad server arguments");
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  if (block != 64 && block != 256) return fail(1, "block must be 64 or 256");
+  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
   if (phases_per_launch < 1 || phases_per_launch > 4096)
     return fail(1, "phases_per_launch must be in [1, 4096]");
   if (int rc = check_params(params, precision)) return rc;

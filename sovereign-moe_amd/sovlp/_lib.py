@@ -47,7 +47,7 @@ class sovlp_info(C.Structure):
                 ("dual_res", F64), ("gap", F64), ("primal_obj", F64), ("dual_obj", F64),
                 ("step_size", F64), ("primal_weight", F64), ("objective", F64),
                 ("scale_obj", F64), ("device_ticks", I64), ("refine_rows", I32),
-                ("refine_round", I32), ("refine_scale", F64)]
+                ("refine_round", I32), ("refine_scale", F64), ("start_tick", I64)]
 
 
 # name -> (restype, argtypes); every entry point of include/sovlp.h
@@ -89,6 +89,8 @@ _SIGS = {
     "sovlp_batch_workspace_bytes": (C.c_int, [C.c_int, P, C.c_int, I64, C.POINTER(I64)]),
     "sovlp_batch_create": (C.c_int, [C.c_int, P, C.POINTER(sovlp_params), C.c_int, C.c_int, P,
                                      I64, I64, P, C.POINTER(P)]),
+    "sovlp_batch_create_mixed": (C.c_int, [C.c_int, P, C.POINTER(sovlp_params), C.c_int, P, P,
+                                           I64, I64, P, C.POINTER(P)]),
     "sovlp_batch_run": (C.c_int, [P, F64, I64, P, C.POINTER(I64)]),
     "sovlp_batch_infos": (C.c_int, [P, P, P]),
     "sovlp_batch_outputs": (C.c_int, [P, C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),

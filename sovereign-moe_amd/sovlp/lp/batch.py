@@ -42,16 +42,29 @@ class DeviceBatch:
     """Uploads many LPs into concatenated device tensors and builds their
     descriptors (minimisation form; costs negated for maximisations)."""
 
-    def __init__(self, problems: list[Problem], precision: int = 32, block: int = 256):
+    def __init__(self, problems: list[Problem], precision: int = 32, block: int = 256,
+                 configs: list | None = None):
+        """``configs``: per LP (block, tile rows, tile nonzeros), e.g. an
+        autotuner's choice; default: ``block`` with its full geometry."""
         import torch
         self.count = len(problems)
-        rows, tnnz = batch_geometry(precision, block)
+        if configs is None:
+            rows, tnnz = batch_geometry(precision, block)
+            configs = [(int(block), rows, tnnz)] * self.count
+        if len(configs) != self.count:
+            raise ValueError("one kernel configuration per LP")
+        for b, r, t in set(configs):
+            rmax, tmax = batch_geometry(precision, b)
+            if not (1 <= r <= rmax and 1 <= t <= tmax):
+                raise ValueError(f"tile geometry ({r} rows, {t} nonzeros) exceeds block {b}'s")
+        self.configs = [tuple(int(v) for v in c) for c in configs]
+        self.blocks = np.array([c[0] for c in self.configs], dtype=np.int32)
         nmax = max(p.n for p in problems)
         v0 = np.random.default_rng(0).standard_normal(nmax)   # each LP uses a prefix
         cat = {k: [] for k in ("rp", "ri", "rx", "cp", "ci", "cx", "c", "rl", "ru", "lo", "hi",
                                "ta", "tt")}
         meta = []
-        for p in problems:
+        for p, (_b, rows, tnnz) in zip(problems, self.configs):
             A = p.A
             flip = p.sense == ObjSense.MAXIMISE
             ta = plan_tiles(A.rp, tnnz, rows)
@@ -110,12 +123,18 @@ class BatchEngine:
     solves every LP from scratch (the same batch can be solved many times)."""
 
     def __init__(self, problems: list[Problem], params: PDLPParams | None = None,
-                 precision: int = 32, block: int = 256, history_cap: int = 1):
+                 precision: int = 32, block: int = 256, history_cap: int = 1,
+                 tuner=None, configs: list | None = None):
+        """``tuner`` (an ``sovlp.lp.autotune.Autotuner``) picks each LP's kernel
+        configuration from its own shape; ``configs`` gives them explicitly;
+        neither: every LP uses ``block`` threads and its full tile geometry."""
         import torch
         self.params = params or PDLPParams()
         get_backend(self.params.device)
         self.precision, self.block = int(precision), int(block)
-        self.db = DeviceBatch(problems, precision, block)
+        if configs is None and tuner is not None:
+            configs = [tuner.choose(p) for p in problems]
+        self.db = DeviceBatch(problems, precision, block, configs)
         self.count = self.db.count
         nbytes = C.c_int64()
         _lib.call("sovlp_batch_workspace_bytes", self.count, self.db.descs, self.precision,
@@ -124,9 +143,9 @@ class BatchEngine:
         self.cparams = self.params.to_c(self.precision)
         self.stream = torch.cuda.current_stream().cuda_stream
         h = C.c_void_p()
-        _lib.call("sovlp_batch_create", self.count, self.db.descs, C.byref(self.cparams),
-                  self.precision, self.block, self.ws.data_ptr(), nbytes.value, int(history_cap),
-                  self.stream, C.byref(h))
+        _lib.call("sovlp_batch_create_mixed", self.count, self.db.descs, C.byref(self.cparams),
+                  self.precision, self.db.blocks.ctypes.data, self.ws.data_ptr(), nbytes.value,
+                  int(history_cap), self.stream, C.byref(h))
         self.h = h
         self.max_graphs = (int(self.params.max_iter) * 61 + 400) // 64 + 16
 
@@ -236,15 +255,16 @@ class BatchEngine:
 
 def solve_batch(problems: list[Problem], params: PDLPParams | None = None, precision: int = 32,
                 block: int = 256, history_cap: int = 1, repeat: int = 1,
-                keep_outputs: bool = True) -> BatchResult:
+                keep_outputs: bool = True, tuner=None) -> BatchResult:
     """Solve every LP of ``problems`` on the GPU in one batch.
 
-    ``repeat`` > 1 solves the same batch again from scratch (timing). Returns
+    ``repeat`` > 1 solves the same batch again from scratch (timing); ``tuner``
+picks each LP's kernel configuration (``sovlp.lp.autotune``). Returns
     per-LP ``Solution`` objects (same fields as ``solve_pdlp``'s, method
     ``"pdlp[gpu,batch]"``) and the batch throughput.
     """
     import torch
-    eng = BatchEngine(problems, params, precision, block, history_cap)
+    eng = BatchEngine(problems, params, precision, block, history_cap, tuner=tuner)
     try:
         best, graphs = None, 0
         for _ in range(max(1, repeat)):
