@@ -497,6 +497,8 @@ struct sovlp_batch {
   long long ticks_set;
   unsigned* hdone;           // [2] pinned host copies of the done counter
   hipEvent_t ev[2];
+  hipStream_t side[2];       // capture branches of the second and third groups
+  hipEvent_t fork, join[2];
 };
 
 namespace {
@@ -730,10 +732,31 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, cons
   h->pcount = pcount;
   h->ticks_set = c0.ticks;
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+  // one launch per group; the groups are independent branches of the graph
+  // (forked from and joined to the capture stream), so they can run side by side
+  int first = -1, nside = 0;
+  for (int g = 0; g < 3; ++g)
+    if (gcount[g] > 0 && first < 0) first = g;
+  HIPCHK(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->join[k], hipEventDisableTiming));
+  }
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-  for (int g = 0, off = 0; g < 3; off += gcount[g], ++g)
-    if (gcount[g] > 0)
+  HIPCHK(hipEventRecord(h->fork, h->cap));
+  for (int g = 0, off = 0; g < 3; off += gcount[g], ++g) {
+    if (gcount[g] == 0) continue;
+    if (g == first) {
       launch_run<T>(kBlocks[g], d_lps + off, gcount[g], ctl, done, kPhasesPerLaunch, h->cap);
+    } else {
+      hipStream_t sd = h->side[nside];
+      HIPCHK(hipStreamWaitEvent(sd, h->fork, 0));
+      launch_run<T>(kBlocks[g], d_lps + off, gcount[g], ctl, done, kPhasesPerLaunch, sd);
+      HIPCHK(hipEventRecord(h->join[nside], sd));
+      ++nside;
+    }
+  }
+  for (int k = 0; k < nside; ++k) HIPCHK(hipStreamWaitEvent(h->cap, h->join[k], 0));
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
   HIPCHK(hipHostMalloc((void**)&h->hdone, 2 * sizeof(unsigned), hipHostMallocDefault));
@@ -1066,6 +1089,11 @@ int sovlp_batch_phase_counts(sovlp_batch* h, int64_t* counts, sovlp_stream_t str
 
 int sovlp_batch_destroy(sovlp_batch* h) {
   if (!h) return 0;
+  for (int k = 0; k < 2; ++k) {
+    if (h->side[k]) (void)hipStreamDestroy(h->side[k]);
+    if (h->join[k]) (void)hipEventDestroy(h->join[k]);
+  }
+  if (h->fork) (void)hipEventDestroy(h->fork);
   for (hipEvent_t e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->hdone) (void)hipHostFree(h->hdone);
