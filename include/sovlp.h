@@ -91,10 +91,12 @@ int sovlp_nrm2_f32(const float *a, int64_t n, double *out, sovlp_stream_t stream
 
 /* Host function.  Cuts rows into tiles of consecutive rows with at most
  * tile_nnz nonzeros and at most tile_rows rows; a row longer than tile_nnz is
- * a tile of its own.  tiles_out needs rows+1 entries; *ntiles_out receives the
- * tile count (tiles_out[0..ntiles] are row starts).  No reference counterpart
- * (backend.py:118-152 is row-per-warp only). */
-int sovlp_plan_tiles(int rows, const int64_t *ptr, int tile_nnz, int tile_rows,
+ * a tile of its own.  With block > 0 (the lanes of the workgroup that will run
+ * the plan) a tile also ends before a row that would leave one lane more than
+ * 64 serial additions of the row reductions.  tiles_out needs rows+1 entries;
+ * *ntiles_out receives the tile count (tiles_out[0..ntiles] are row starts).
+ * No reference counterpart (backend.py:118-152 is row-per-warp only). */
+int sovlp_plan_tiles(int rows, const int64_t *ptr, int tile_nnz, int tile_rows, int block,
                      int32_t *tiles_out, int32_t *ntiles_out);
 
 /* Host function.  Splits ntiles tiles into `parts` contiguous ranges of equal
@@ -103,8 +105,9 @@ int sovlp_partition_tiles(int ntiles, const int32_t *tiles, const int64_t *ptr, 
                           int32_t *part_out);
 
 /* Tile geometry compiled into the single-LP engine and the tiled SpMV: the
- * row cap and nonzero cap of a tile (plan tiles with these). */
-int sovlp_tile_geometry(int precision, int *tile_rows, int *tile_nnz);
+ * workgroup size, and the row cap and nonzero cap of a tile (plan tiles with
+ * these). */
+int sovlp_tile_geometry(int precision, int *block, int *tile_rows, int *tile_nnz);
 
 /* y = A x through the engines' tile pass (ptr32 = int32 row pointers).
  * `part` has grid+1 entries (sovlp_partition_tiles with parts = grid). */

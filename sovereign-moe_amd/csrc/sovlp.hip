@@ -268,8 +268,9 @@ int sovlp_clock_rate(int device, double* hz) {
   return 0;
 }
 
-int sovlp_tile_geometry(int precision, int* tile_rows, int* tile_nnz) {
+int sovlp_tile_geometry(int precision, int* block, int* tile_rows, int* tile_nnz) {
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  *block = kBlock;
   *tile_rows = kRowsPerLane * kBlock;
   *tile_nnz = precision == 64 ? kTnnz64 : kTnnz32;
   return 0;
@@ -332,10 +333,28 @@ int sovlp_nrm2_f32(const float* a, int64_t n, double* out, sovlp_stream_t s) {
   return launch_reduce<float, 1>(a, a, n, out, S(s));
 }
 
-int sovlp_plan_tiles(int rows, const int64_t* ptr, int tile_nnz, int tile_rows,
+// Serial additions one lane makes to reduce the rows of a tile with `rows`
+// rows, the longest of length `maxlen`, on `block` lanes (tile_pass_pf: up
+// to 64 lanes per row when the rows fit the lanes, else one lane for up to
+// kRowsPerLane rows).
+static int64_t tile_serial_work(int64_t rows, int64_t maxlen, int block) {
+  if (rows <= block) {
+    int tpr = 64;
+    while (tpr > 1 && (int64_t)tpr * rows > block) tpr >>= 1;
+    return (maxlen + tpr - 1) / tpr;
+  }
+  return ((rows + block - 1) / block) * maxlen;
+}
+
+int sovlp_plan_tiles(int rows, const int64_t* ptr, int tile_nnz, int tile_rows, int block,
                      int32_t* tiles, int32_t* ntiles_out) {
-  if (rows < 0 || tile_nnz < 1 || tile_rows < 1) return fail(1, "bad tile plan arguments");
+  if (rows < 0 || tile_nnz < 1 || tile_rows < 1 || block < 0)
+    return fail(1, "bad tile plan arguments");
   if (ptr[rows] >= (int64_t)INT32_MAX) return fail(4, "more than 2^31-1 nonzeros");
+  // a tile also ends before a row that would make one lane's serial share of
+  // the row reductions exceed kMaxSerial additions (a long row among many
+  // short ones would otherwise be summed by a single lane)
+  constexpr int64_t kMaxSerial = 64;
   int t = 0, r = 0;
   tiles[0] = 0;
   while (r < rows) {
@@ -345,12 +364,15 @@ int sovlp_plan_tiles(int rows, const int64_t* ptr, int tile_nnz, int tile_rows,
       tiles[++t] = ++r;
       continue;
     }
-    int64_t acc = 0;
+    int64_t acc = 0, maxlen = 0;
     const int start = r;
     while (r < rows && r - start < tile_rows) {
       const int64_t l = ptr[r + 1] - ptr[r];
       if (l > tile_nnz || acc + l > tile_nnz) break;
+      const int64_t ml = l > maxlen ? l : maxlen;
+      if (block > 0 && r > start && tile_serial_work(r - start + 1, ml, block) > kMaxSerial) break;
       acc += l;
+      maxlen = ml;
       ++r;
     }
     tiles[++t] = r;

@@ -68,9 +68,10 @@ _SIGS = {
     "sovlp_nrminf_f64": (C.c_int, [P, I64, P, P]),
     "sovlp_dot_f32": (C.c_int, [P, P, I64, P, P]),
     "sovlp_nrm2_f32": (C.c_int, [P, I64, P, P]),
-    "sovlp_plan_tiles": (C.c_int, [C.c_int, P, C.c_int, C.c_int, P, C.POINTER(I32)]),
+    "sovlp_plan_tiles": (C.c_int, [C.c_int, P, C.c_int, C.c_int, C.c_int, P, C.POINTER(I32)]),
     "sovlp_partition_tiles": (C.c_int, [C.c_int, P, P, C.c_int, P]),
-    "sovlp_tile_geometry": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sovlp_tile_geometry": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "sovlp_spmv_tiled_f64": (C.c_int, [C.c_int, P, P, P, P, C.c_int, P, C.c_int, P, P, P]),
     "sovlp_spmv_tiled_f32": (C.c_int, [C.c_int, P, P, P, P, C.c_int, P, C.c_int, P, P, P]),
     "sovlp_pdlp_workspace_bytes": (C.c_int, [C.POINTER(sovlp_lp), C.c_int, I64, C.POINTER(I64)]),

@@ -64,11 +64,11 @@ class DeviceBatch:
         cat = {k: [] for k in ("rp", "ri", "rx", "cp", "ci", "cx", "c", "rl", "ru", "lo", "hi",
                                "ta", "tt")}
         meta = []
-        for p, (_b, rows, tnnz) in zip(problems, self.configs):
+        for p, (blk, rows, tnnz) in zip(problems, self.configs):
             A = p.A
             flip = p.sense == ObjSense.MAXIMISE
-            ta = plan_tiles(A.rp, tnnz, rows)
-            tt = plan_tiles(A.cp, tnnz, rows)
+            ta = plan_tiles(A.rp, tnnz, rows, blk)
+            tt = plan_tiles(A.cp, tnnz, rows, blk)
             cat["rp"].append(A.rp.astype(np.int32))
             cat["ri"].append(A.ri)
             cat["rx"].append(A.rx)

@@ -104,21 +104,22 @@ class PDLPInfo:
 _TEAMS = {"auto": 0, "block": 1, "grid": 2}
 
 
-def tile_geometry(precision: int) -> tuple[int, int]:
-    """(tile row cap, tile nonzero cap) of the single-LP engine."""
-    r, t = C.c_int(), C.c_int()
-    _lib.call("sovlp_tile_geometry", int(precision), C.byref(r), C.byref(t))
-    return r.value, t.value
+def tile_geometry(precision: int) -> tuple[int, int, int]:
+    """(workgroup size, tile row cap, tile nonzero cap) of the single-LP engine."""
+    b, r, t = C.c_int(), C.c_int(), C.c_int()
+    _lib.call("sovlp_tile_geometry", int(precision), C.byref(b), C.byref(r), C.byref(t))
+    return b.value, r.value, t.value
 
 
-def plan_tiles(ptr: np.ndarray, tile_nnz: int, tile_rows: int) -> np.ndarray:
-    """Host tile plan of a CSR pointer array (``sovlp_plan_tiles``)."""
+def plan_tiles(ptr: np.ndarray, tile_nnz: int, tile_rows: int, block: int = 0) -> np.ndarray:
+    """Host tile plan of a CSR pointer array (``sovlp_plan_tiles``); ``block``:
+    lanes of the workgroup that runs it (0: no cap on a lane's serial work)."""
     ptr = np.ascontiguousarray(ptr, dtype=np.int64)
     rows = ptr.size - 1
     out = np.zeros(rows + 1, dtype=np.int32)
     nt = C.c_int32()
     _lib.call("sovlp_plan_tiles", rows, ptr.ctypes.data, int(tile_nnz), int(tile_rows),
-              out.ctypes.data, C.byref(nt))
+              int(block), out.ctypes.data, C.byref(nt))
     return out[: nt.value + 1].copy()
 
 
@@ -127,19 +128,19 @@ class DeviceLP:
     ``sovlp_lp`` descriptor pointing at them. Keeps the tensors alive."""
 
     def __init__(self, prob: Problem, precision: int = 64,
-                 geometry: tuple[int, int] | None = None, v0=None):
-        """``geometry`` = (tile rows, tile nonzeros) of the engine that will run
-        the LP (default: the single-LP engine's); ``v0`` a device tensor whose
-        first ``n`` entries are ``default_rng(0).standard_normal(n)`` (default:
-        a fresh one)."""
+                 geometry: tuple[int, int, int] | None = None, v0=None):
+        """``geometry`` = (workgroup size, tile rows, tile nonzeros) of the engine
+        that will run the LP (default: the single-LP engine's); ``v0`` a device
+        tensor whose first ``n`` entries are ``default_rng(0).standard_normal(n)``
+        (default: a fresh one)."""
         import torch
         flip = prob.sense == ObjSense.MAXIMISE
         A = prob.A
         if A.nnz >= 2**31 - 1:
             raise ValueError("more than 2^31-1 nonzeros")
-        block, tnnz = geometry if geometry is not None else tile_geometry(precision)
-        self.tiles_a = plan_tiles(A.rp, tnnz, block)
-        self.tiles_t = plan_tiles(A.cp, tnnz, block)
+        block, rows, tnnz = geometry if geometry is not None else tile_geometry(precision)
+        self.tiles_a = plan_tiles(A.rp, tnnz, rows, block)
+        self.tiles_t = plan_tiles(A.cp, tnnz, rows, block)
         dev = "cuda"
 
         def up(a, dt):

@@ -141,7 +141,7 @@ class DeviceServer:
         self.slots = int(slots)
         self.precision, self.block = int(precision), int(block)
         self.history_cap = int(history_cap)
-        self.geometry = batch_geometry(self.precision, self.block)
+        self.geometry = (self.block,) + tuple(batch_geometry(self.precision, self.block))
         nb = C.c_int64()
         _lib.call("sovlp_server_workspace_bytes", self.slots, self.precision, C.byref(nb))
         self.table = torch.empty(int(nb.value), dtype=torch.uint8, device="cuda")
@@ -289,6 +289,8 @@ class ServeLoop:
     warm_starts: bool = True
     fallback: object = None
     warm: dict = field(default_factory=dict)
+    progress: object = None       # called with a status dict every progress_every seconds
+    progress_every: float = 2.0
 
     def _admit(self, lane: _Lane, t0: float, out: list) -> None:
         adm = self.admission
@@ -324,8 +326,15 @@ class ServeLoop:
         out: list[Served] = []
         i = 0
         t0 = self.clock()
+        next_report = self.progress_every
         while i < len(order) or any(ln.queue or ln.active for ln in lanes):
             now = self.clock() - t0
+            if self.progress is not None and now >= next_report:
+                next_report = now + self.progress_every
+                self.progress(dict(t=now, arrived=i, finished=len(out),
+                                   queued=[len(ln.queue) for ln in lanes],
+                                   running=[len(ln.active) for ln in lanes],
+                                   hbm_used=adm.used))
             while i < len(order) and order[i].arrival <= now:
                 r = order[i]
                 w = self.warm.get(r.model) if (self.warm_starts and r.resolve) else None

@@ -33,8 +33,8 @@ def measure(ptr: np.ndarray, idx: np.ndarray, val: np.ndarray, ncols: int, preci
     import torch
     rows = ptr.size - 1
     nnz = int(ptr[-1])
-    block, tnnz = tile_geometry(precision)
-    tiles = plan_tiles(ptr, tnnz, block)
+    block, trows, tnnz = tile_geometry(precision)
+    tiles = plan_tiles(ptr, tnnz, trows, block)
     nt = tiles.size - 1
     if grid is None:
         props = torch.cuda.get_device_properties(0)

@@ -97,8 +97,8 @@ def _tiled(A_ptr, A_idx, A_val, x, rows, dtype, grid=7):
     from sovlp import _lib
     from sovlp.lp.pdlp import plan_tiles, tile_geometry
     prec = 64 if dtype == np.float64 else 32
-    block, tnnz = tile_geometry(prec)
-    tiles = plan_tiles(A_ptr, tnnz, block)
+    block, trows, tnnz = tile_geometry(prec)
+    tiles = plan_tiles(A_ptr, tnnz, trows, block)
     nt = tiles.size - 1
     part = np.zeros(grid + 1, dtype=np.int32)
     _lib.call("sovlp_partition_tiles", nt, tiles.ctypes.data,
