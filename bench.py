@@ -73,7 +73,7 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int):
+def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int, tuned: bool):
     """HBM bytes per launch of ``kernel`` on this workload from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's reading;
     see profiles/), or None when they were taken on another configuration."""
@@ -83,7 +83,8 @@ def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: in
     except (OSError, ValueError):
         return None
     want = f"{lps} uneven LPs, seeds {seed}..{seed + lps - 1}, " \
-           f"{'FP32 + refinement' if precision == 32 else 'FP64'}, block {block}"
+           f"{'FP32 + refinement' if precision == 32 else 'FP64'}, block {block}" \
+           f"{', autotuned' if tuned else ''}"
     if d.get("kernel") != kernel or d.get("workload") != want:
         return None
     return float(d["traffic_bytes_per_launch"])
@@ -276,7 +277,8 @@ def main():
         "escalation_s_per_step": esc_s / a.steps,
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kernel, a.lps, a.seed, a.precision, a.block),
+                     "traffic": measured_traffic(kernel, a.lps, a.seed, a.precision, a.block,
+                                                 tuner is not None),
                      "algorithmic_bytes_per_launch": step_bytes / graphs[-1],
                      "avg_launch_us": launch_ms * 1e3, "launches": launches},
     }

@@ -305,6 +305,9 @@ typedef struct sovlp_server sovlp_server;
 /* Bytes of the server's own device table for `slots` slots. */
 int sovlp_server_workspace_bytes(int slots, int precision, int64_t *bytes);
 
+/* The same for a team server (sovlp_server_create_team). */
+int sovlp_server_team_workspace_bytes(int slots, int team, int precision, int64_t *bytes);
+
 /* Workspace bytes of one admitted LP (history_cap history rows). */
 int sovlp_server_lp_bytes(const sovlp_lp *lp, int precision, int64_t history_cap,
                           int64_t *bytes);
@@ -315,6 +318,18 @@ int sovlp_server_lp_bytes(const sovlp_lp *lp, int precision, int64_t history_cap
 int sovlp_server_create(int slots, const sovlp_params *params, int precision, int block,
                         int phases_per_launch, void *workspace, int64_t workspace_bytes,
                         sovlp_stream_t stream, sovlp_server **out);
+
+/* A server whose slots run each phase over `team` workgroups of 256 threads
+ * (for LPs of tens of thousands of nonzeros and more, whose phases one
+ * workgroup would run tile after tile): a replay is `pairs_per_launch` pairs
+ * of launches, one phase of every occupied slot over slots x team
+ * workgroups, then one workgroup per slot that reduces the team's partial
+ * sums in a fixed order and takes the transition.  An LP's result depends on
+ * its data and `team` only.  Admission plans the team's tile split on the
+ * host (it synchronises the stream once).  Synchronises. */
+int sovlp_server_create_team(int slots, int team, const sovlp_params *params, int precision,
+                             int pairs_per_launch, void *workspace, int64_t workspace_bytes,
+                             sovlp_stream_t stream, sovlp_server **out);
 
 /* Admits an LP into free slot `slot` (asynchronous, stream-ordered before
  * the next step).  Warm start: x0 (n) and y0 (m) are device vectors in the

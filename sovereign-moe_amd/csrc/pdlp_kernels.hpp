@@ -193,3 +193,55 @@ __global__ void state_reset_kernel(const LpDev<T>* __restrict__ lps, int count, 
 
 
 }  // namespace sov
+
+namespace sov {
+
+// Served LPs over `team` workgroups each (slot = blockIdx.x / team, rank =
+// blockIdx.x % team): one phase of every occupied slot, partial sums per rank.
+template <class T, int BLOCK, int TNNZ>
+__global__ void __launch_bounds__(BLOCK)
+    pdlp_slot_phase_kernel(const LpDev<T>* __restrict__ lps, int team) {
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
+  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
+  const int slot = (int)blockIdx.x / team, rank = (int)blockIdx.x % team;
+  const LpDev<T>& L = lps[slot];
+  const LpState s = *L.st;
+  const int ph = uniform(s.phase);
+  if (ph == PH_DONE) return;
+  double part[kMaxRed];
+  init_part(ph, part);
+  PhaseWork<T, BLOCK, TNNZ> W(L, s, reinterpret_cast<T*>(lds_raw), rank, team);
+  W.run(ph, part);
+  reduce_phase_to_lds<BLOCK>(ph, part, red);
+  constexpr int NW = BLOCK / kWave;
+  if (threadIdx.x < kMaxRed)
+    L.partials[(size_t)rank * kMaxRed + threadIdx.x] = red[NW * kMaxRed + threadIdx.x];
+}
+
+// One workgroup per slot: reduce the team's partials (fixed order) and take
+// the transition.
+template <class T, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+    pdlp_slot_decide_kernel(const LpDev<T>* __restrict__ lps, int team,
+                            const StepCtl* __restrict__ ctl, unsigned* done) {
+  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
+  __shared__ LpState ss;
+  const LpDev<T>& L = lps[blockIdx.x];
+  const LpState s = *L.st;
+  const int ph = uniform(s.phase);
+  if (ph == PH_DONE) return;
+  const uint32_t mm = phase_maxmask(ph);
+  double part[kMaxRed];
+#pragma unroll
+  for (int k = 0; k < kMaxRed; ++k) part[k] = ((mm >> k) & 1u) ? -__builtin_inf() : 0.0;
+  for (int b = threadIdx.x; b < team; b += BLOCK) {
+    const double* pb = L.partials + (size_t)b * kMaxRed;
+#pragma unroll
+    for (int k = 0; k < kMaxRed; ++k)
+      part[k] = ((mm >> k) & 1u) ? nanmax(part[k], pb[k]) : part[k] + pb[k];
+  }
+  block_reduce_to_lds<BLOCK, kMaxRed>(part, mm, red);
+  if (threadIdx.x == 0) take_transition(L, s, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
+}
+
+}  // namespace sov

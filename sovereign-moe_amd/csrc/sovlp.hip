@@ -1138,6 +1138,10 @@ int sovlp_batch_destroy(sovlp_batch* h) {
 // their workgroups return at once and never touch a released LP's memory.
 struct sovlp_server {
   int slots, precision, block, phases;
+  int team;                  // workgroups per slot and phase (1: run kernel)
+  double* partials;          // [slots][team][kMaxRed] (team > 1)
+  int32_t* parts;            // [slots][2][team + 1] tile partitions of A and A^T (team > 1)
+  int32_t* hparts;           // pinned staging of the partitions
   void* lps;                 // LpDev<T>[slots] (device)
   LpState* states;           // [slots] (device)
   long long* pcount;         // [slots][kPhaseSlots] (device)
@@ -1157,13 +1161,17 @@ struct sovlp_server {
 namespace {
 
 template <class T>
-size_t server_ws_T(int slots) {
+size_t server_ws_T(int slots, int team) {
   Carver cv{nullptr, 0, 0};
   cv.take<LpDev<T>>(slots);
   cv.take<LpState>(slots);
   cv.take<long long>((size_t)slots * kPhaseSlots);
   cv.take<StepCtl>(1);
   cv.take<unsigned>(64);
+  if (team > 1) {
+    cv.take<double>((size_t)slots * team * kMaxRed);
+    cv.take<int32_t>((size_t)slots * 2 * (team + 1));
+  }
   return cv.off + 256;
 }
 
@@ -1176,15 +1184,22 @@ size_t server_lp_ws_T(const sovlp_lp* lp, long long hist_cap) {
 }
 
 template <class T>
-int server_create_T(int slots, const sovlp_params* prm, int block, int phases, void* ws,
+int server_create_T(int slots, int team, const sovlp_params* prm, int block, int phases, void* ws,
                     int64_t ws_bytes, hipStream_t st, sovlp_server* h) {
-  if ((size_t)ws_bytes < server_ws_T<T>(slots)) return fail(1, "workspace too small");
+  if ((size_t)ws_bytes < server_ws_T<T>(slots, team)) return fail(1, "workspace too small");
   Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
   LpDev<T>* d_lps = cv.take<LpDev<T>>(slots);
   LpState* states = cv.take<LpState>(slots);
   long long* pcount = cv.take<long long>((size_t)slots * kPhaseSlots);
   StepCtl* ctl = cv.take<StepCtl>(1);
   unsigned* done = cv.take<unsigned>(64);
+  h->team = team;
+  if (team > 1) {
+    h->partials = cv.take<double>((size_t)slots * team * kMaxRed);
+    h->parts = cv.take<int32_t>((size_t)slots * 2 * (team + 1));
+    HIPCHK(hipHostMalloc((void**)&h->hparts, sizeof(int32_t) * (size_t)slots * 2 * (team + 1),
+                         hipHostMallocDefault));
+  }
   h->slots = slots;
   h->precision = sizeof(T) == 8 ? 64 : 32;
   h->block = block;
@@ -1227,7 +1242,17 @@ int server_create_T(int slots, const sovlp_params* prm, int block, int phases, v
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-  launch_run<T>(block, d_lps, slots, ctl, done, phases, h->cap);
+  if (team > 1) {
+    // `phases` pairs: one phase of every slot over `team` workgroups, then one
+    // workgroup per slot reduces the partial sums and takes the transition
+    constexpr int TN = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
+    for (int i = 0; i < phases; ++i) {
+      pdlp_slot_phase_kernel<T, kBlock, TN><<<slots * team, kBlock, 0, h->cap>>>(d_lps, team);
+      pdlp_slot_decide_kernel<T, kBlock><<<slots, kBlock, 0, h->cap>>>(d_lps, team, ctl, done);
+    }
+  } else {
+    launch_run<T>(block, d_lps, slots, ctl, done, phases, h->cap);
+  }
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
   return 0;
@@ -1244,6 +1269,28 @@ int server_admit_T(sovlp_server* h, int slot, const sovlp_lp* lp, const double* 
   fill_lp<T>(L, *lp);
   L.st = h->states + slot;   // the state lives in the server's table, not the LP's workspace
   L.pcount = h->pcount + (size_t)slot * kPhaseSlots;
+  if (h->team > 1) {
+    // equal-cost split of both tile lists over the team (host plan from the
+    // device copies of the tile starts and row pointers)
+    const int W = h->team;
+    std::vector<int32_t> ta(lp->ntiles_a + 1), tt(lp->ntiles_t + 1), rp(lp->m + 1), cp(lp->n + 1);
+    HIPCHK(hipMemcpyAsync(ta.data(), lp->tiles_a, sizeof(int32_t) * ta.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(tt.data(), lp->tiles_t, sizeof(int32_t) * tt.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(rp.data(), lp->rp, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cp.data(), lp->cp, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int64_t> pa(ta.size()), pt(tt.size());
+    for (size_t t = 0; t < ta.size(); ++t) pa[t] = rp[ta[t]];
+    for (size_t t = 0; t < tt.size(); ++t) pt[t] = cp[tt[t]];
+    int32_t* hp = h->hparts + (size_t)slot * 2 * (W + 1);
+    host_partition(lp->ntiles_a, ta.data(), pa.data(), W, hp);
+    host_partition(lp->ntiles_t, tt.data(), pt.data(), W, hp + W + 1);
+    int32_t* dp = h->parts + (size_t)slot * 2 * (W + 1);
+    HIPCHK(hipMemcpyAsync(dp, hp, sizeof(int32_t) * 2 * (W + 1), hipMemcpyHostToDevice, st));
+    L.partA = dp;
+    L.partT = dp + W + 1;
+    L.partials = h->partials + (size_t)slot * W * kMaxRed;
+  }
   L.wx0 = x0;
   L.wy0 = y0;
   L.womega = omega0 > 0.0 ? omega0 : 0.0;
@@ -1268,9 +1315,14 @@ int server_admit_T(sovlp_server* h, int slot, const sovlp_lp* lp, const double* 
 extern "C" {
 
 int sovlp_server_workspace_bytes(int slots, int precision, int64_t* bytes) {
-  if (slots < 1 || !bytes) return fail(1, "slots must be >= 1");
+  return sovlp_server_team_workspace_bytes(slots, 1, precision, bytes);
+}
+
+int sovlp_server_team_workspace_bytes(int slots, int team, int precision, int64_t* bytes) {
+  if (slots < 1 || team < 1 || !bytes) return fail(1, "slots and team must be >= 1");
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  *bytes = (int64_t)(precision == 64 ? server_ws_T<double>(slots) : server_ws_T<float>(slots));
+  *bytes = (int64_t)(precision == 64 ? server_ws_T<double>(slots, team)
+                                     : server_ws_T<float>(slots, team));
   return 0;
 }
 
@@ -1284,20 +1336,22 @@ int sovlp_server_lp_bytes(const sovlp_lp* lp, int precision, int64_t history_cap
   return 0;
 }
 
-int sovlp_server_create(int slots, const sovlp_params* params, int precision, int block,
-                        int phases_per_launch, void* workspace, int64_t workspace_bytes,
-                        sovlp_stream_t stream, sovlp_server** out) {
-  if (slots < 1 || !params || !workspace || !out) return fail(1, "bad server arguments");
+static int server_create(int slots, int team, const sovlp_params* params, int precision,
+                         int block, int phases_per_launch, void* workspace,
+                         int64_t workspace_bytes, sovlp_stream_t stream, sovlp_server** out) {
+  if (slots < 1 || team < 1 || !params || !workspace || !out)
+    return fail(1, "bad server arguments");
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
   if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
   if (phases_per_launch < 1 || phases_per_launch > 4096)
     return fail(1, "phases_per_launch must be in [1, 4096]");
+  if ((long long)slots * team > (1LL << 24)) return fail(1, "slots x team too large");
   if (int rc = check_params(params, precision)) return rc;
   sovlp_server* h = new sovlp_server{};
   int rc = precision == 64
-               ? server_create_T<double>(slots, params, block, phases_per_launch, workspace,
+               ? server_create_T<double>(slots, team, params, block, phases_per_launch, workspace,
                                          workspace_bytes, S(stream), h)
-               : server_create_T<float>(slots, params, block, phases_per_launch, workspace,
+               : server_create_T<float>(slots, team, params, block, phases_per_launch, workspace,
                                         workspace_bytes, S(stream), h);
   if (rc) {
     sovlp_server_destroy(h);
@@ -1305,6 +1359,20 @@ int sovlp_server_create(int slots, const sovlp_params* params, int precision, in
   }
   *out = h;
   return 0;
+}
+
+int sovlp_server_create(int slots, const sovlp_params* params, int precision, int block,
+                        int phases_per_launch, void* workspace, int64_t workspace_bytes,
+                        sovlp_stream_t stream, sovlp_server** out) {
+  return server_create(slots, 1, params, precision, block, phases_per_launch, workspace,
+                       workspace_bytes, stream, out);
+}
+
+int sovlp_server_create_team(int slots, int team, const sovlp_params* params, int precision,
+                             int pairs_per_launch, void* workspace, int64_t workspace_bytes,
+                             sovlp_stream_t stream, sovlp_server** out) {
+  return server_create(slots, team, params, precision, kBlock, pairs_per_launch, workspace,
+                       workspace_bytes, stream, out);
 }
 
 int sovlp_server_admit(sovlp_server* h, int slot, const sovlp_lp* lp, const double* x0,
@@ -1402,6 +1470,7 @@ int sovlp_server_destroy(sovlp_server* h) {
   if (h->hstage) (void)hipHostFree(h->hstage);
   if (h->hinit) (void)hipHostFree(h->hinit);
   if (h->hsnap) (void)hipHostFree(h->hsnap);
+  if (h->hparts) (void)hipHostFree(h->hparts);
   delete h;
   return 0;
 }

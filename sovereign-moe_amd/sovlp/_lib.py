@@ -99,7 +99,10 @@ _SIGS = {
     "sovlp_batch_phase_counts": (C.c_int, [P, P, P]),
     "sovlp_batch_destroy": (C.c_int, [P]),
     "sovlp_server_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.POINTER(I64)]),
+    "sovlp_server_team_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(I64)]),
     "sovlp_server_lp_bytes": (C.c_int, [C.POINTER(sovlp_lp), C.c_int, I64, C.POINTER(I64)]),
+    "sovlp_server_create_team": (C.c_int, [C.c_int, C.c_int, C.POINTER(sovlp_params), C.c_int,
+                                           C.c_int, P, I64, P, C.POINTER(P)]),
     "sovlp_server_create": (C.c_int, [C.c_int, C.POINTER(sovlp_params), C.c_int, C.c_int,
                                       C.c_int, P, I64, P, C.POINTER(P)]),
     "sovlp_server_admit": (C.c_int, [P, C.c_int, C.POINTER(sovlp_lp), P, P, F64, P, I64, I64,

@@ -134,23 +134,35 @@ class DeviceServer:
     is torch's; each request's tensors live until its slot is released)."""
 
     def __init__(self, slots: int, params: PDLPParams | None = None, precision: int = 32,
-                 block: int = 256, phases_per_launch: int = 32, history_cap: int = 1):
+                 block: int = 256, phases_per_launch: int = 32, history_cap: int = 1,
+                 team: int = 1):
+        """``team`` > 1: every slot runs each phase over ``team`` workgroups of
+        256 threads (``sovlp_server_create_team``; ``phases_per_launch`` phases
+        per replay as launch pairs), for LPs too large for one workgroup."""
         import torch
         self.params = params or PDLPParams()
         get_backend(self.params.device)
-        self.slots = int(slots)
+        self.slots, self.team = int(slots), int(team)
+        if self.team > 1:
+            block = 256
         self.precision, self.block = int(precision), int(block)
         self.history_cap = int(history_cap)
         self.geometry = (self.block,) + tuple(batch_geometry(self.precision, self.block))
         nb = C.c_int64()
-        _lib.call("sovlp_server_workspace_bytes", self.slots, self.precision, C.byref(nb))
+        _lib.call("sovlp_server_team_workspace_bytes", self.slots, self.team, self.precision,
+                  C.byref(nb))
         self.table = torch.empty(int(nb.value), dtype=torch.uint8, device="cuda")
         self.cparams = self.params.to_c(self.precision)
         self.stream = torch.cuda.current_stream().cuda_stream
         h = C.c_void_p()
-        _lib.call("sovlp_server_create", self.slots, C.byref(self.cparams), self.precision,
-                  self.block, int(phases_per_launch), self.table.data_ptr(), nb.value,
-                  self.stream, C.byref(h))
+        if self.team > 1:
+            _lib.call("sovlp_server_create_team", self.slots, self.team, C.byref(self.cparams),
+                      self.precision, int(phases_per_launch), self.table.data_ptr(), nb.value,
+                      self.stream, C.byref(h))
+        else:
+            _lib.call("sovlp_server_create", self.slots, C.byref(self.cparams), self.precision,
+                      self.block, int(phases_per_launch), self.table.data_ptr(), nb.value,
+                      self.stream, C.byref(h))
         self.h = h
         hz = C.c_double()
         _lib.call("sovlp_server_clock_hz", self.h, C.byref(hz))

@@ -58,8 +58,10 @@ def main():
     ap.add_argument("--resolve", type=float, default=0.3)
     ap.add_argument("--tmin", type=int, default=24)
     ap.add_argument("--tmax", type=int, default=36)
-    ap.add_argument("--slots", type=int, default=512)
-    ap.add_argument("--fallback-slots", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=128)
+    ap.add_argument("--team", type=int, default=8,
+                    help="workgroups per served LP and phase (1: one workgroup runs all phases)")
+    ap.add_argument("--fallback-slots", type=int, default=32)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--hbm-fraction", type=float, default=0.8)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64),
@@ -81,10 +83,11 @@ def main():
     par64 = PDLPParams(eps_abs=a.tol, eps_rel=a.tol, feas_cap=a.tol, device="gpu", precision=64,
                        time_limit=a.time_limit)
     if a.precision == 32:
-        srv = DeviceServer(a.slots, par32, precision=32, phases_per_launch=32)
-        fb = DeviceServer(a.fallback_slots, par64, precision=64, phases_per_launch=32)
+        srv = DeviceServer(a.slots, par32, precision=32, phases_per_launch=32, team=a.team)
+        fb = DeviceServer(a.fallback_slots, par64, precision=64, phases_per_launch=32,
+                          team=a.team)
     else:
-        srv = DeviceServer(a.slots, par64, precision=64, phases_per_launch=32)
+        srv = DeviceServer(a.slots, par64, precision=64, phases_per_launch=32, team=a.team)
         fb = None
     adm = HbmAdmission(hbm_budget(a.hbm_fraction))
     try:
@@ -104,7 +107,8 @@ def main():
                             f"P 5-20, T {a.tmin}-{a.tmax}, {a.resolve:.0%} warm-started resolves",
                    wall_s=wall, nnz_mean=float(nnz.mean()), nnz_max=int(nnz.max()),
                    hbm_budget_bytes=adm.budget, hbm_peak_bytes=adm.peak, slots=a.slots,
-                   fallback_slots=a.fallback_slots if fb is not None else 0, tolerance=a.tol,
+                   fallback_slots=a.fallback_slots if fb is not None else 0, team=a.team,
+                   tolerance=a.tol,
                    precision=a.precision, time_limit_s=a.time_limit))
     print(json.dumps(st), flush=True)
 

@@ -65,12 +65,13 @@ def _same(a, b):
     assert np.array_equal(a.reduced_costs, b.reduced_costs)
 
 
-@pytest.mark.parametrize("precision", [32, 64])
-def test_served_result_is_identical_alone_and_in_the_mix(precision):
+@pytest.mark.parametrize("precision,team", [(32, 1), (64, 1), (32, 4), (64, 3)])
+def test_served_result_is_identical_alone_and_in_the_mix(precision, team):
+    print(f"[gpu] served alone vs mix, precision {precision}, team {team}", flush=True)
     target = build_blend(BLEND["cases"][2])
     others = [uneven_batch_member(5000 + i).problem for i in range(5)] + \
              [build_blend(c) for c in BLEND["cases"][4:8]]
-    srv = DeviceServer(4, _params(), precision=precision, phases_per_launch=16)
+    srv = DeviceServer(4, _params(), precision=precision, phases_per_launch=16, team=team)
     try:
         alone = _solve_in(srv, [[("t", target, None)]])["t"]
         # in the mix: three other LPs take slots 0-2, the target joins slot 3 three
@@ -84,6 +85,8 @@ def test_served_result_is_identical_alone_and_in_the_mix(precision):
         srv.close()
     assert int(alone.status) == int(Status.OPTIMAL)
     _same(alone, mixed)
+    f = BLEND["cases"][2]["objective"]
+    assert abs(alone.objective - f) <= 1e-6 * max(1.0, abs(f))
 
 
 def test_served_blending_lps_and_resolves_match_highs():
