@@ -122,6 +122,47 @@ __device__ __forceinline__ void block_combine(T (&s)[NV], T* scratch) {
 // BLOCK rows (the host plans tiles with that row cap).
 constexpr int kRowsPerLane = 4;
 
+// Nonzeros per lane whose loads are issued together: the column indices of a
+// chunk, then its products (values and gathered entries), then the LDS
+// stores.  A plain loop over the nonzeros would wait for each index before its
+// gather, two dependent memory round trips per nonzero.
+constexpr int kChunk = 8;
+
+// Stage the products of nonzeros [p0, p0 + nz) of a tile (nz <= TNNZ) into
+// LDS slot q = nonzero - p0.  Lanes past the end load the tile's last nonzero
+// (always valid) and discard it.
+template <int BLOCK, int TNNZ, int NV, class T, class Prod>
+__device__ __forceinline__ void stage_products(const int32_t* __restrict__ idx, int p0, int nz,
+                                               Prod& prod, T* lds) {
+  constexpr int PER = (TNNZ + BLOCK - 1) / BLOCK;
+  constexpr int CH = PER < kChunk ? PER : kChunk;
+  if (nz <= 0) return;
+#pragma unroll
+  for (int c = 0; c < PER; c += CH) {
+    if (c * BLOCK >= nz) break;          // nz is uniform: a scalar branch
+    int jj[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int q = (int)threadIdx.x + (c + k) * BLOCK;
+      jj[k] = idx[p0 + (q < nz ? q : nz - 1)];
+    }
+    T v[CH][NV];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int q = (int)threadIdx.x + (c + k) * BLOCK;
+      prod(p0 + (q < nz ? q : nz - 1), jj[k], v[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int q = (int)threadIdx.x + (c + k) * BLOCK;
+      if (q < nz) {
+#pragma unroll
+        for (int u = 0; u < NV; ++u) lds[u * TNNZ + q] = v[k][u];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------------
 // CSR-Adaptive tile pass.
 //   ptr   : row pointers (LP-local int32 offsets), rows+1 entries
@@ -159,11 +200,28 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
       T s[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k) s[k] = T(0);
-      for (int q = p0 + (int)threadIdx.x; q < p1; q += BLOCK) {
-        T v[NV];
-        prod(q, idx[q], v);
+      // lane order q = p0 + tid, + BLOCK, ...; kChunk of them in flight at once
+      for (int base = p0; base < p1; base += kChunk * BLOCK) {
+        int jj[kChunk];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) s[k] = MAXOP ? tmax(s[k], v[k]) : s[k] + v[k];
+        for (int c = 0; c < kChunk; ++c) {
+          const int q = base + (int)threadIdx.x + c * BLOCK;
+          jj[c] = idx[q < p1 ? q : p1 - 1];
+        }
+        T v[kChunk][NV];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+          const int q = base + (int)threadIdx.x + c * BLOCK;
+          prod(q < p1 ? q : p1 - 1, jj[c], v[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+          const int q = base + (int)threadIdx.x + c * BLOCK;
+          if (q < p1) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) s[k] = MAXOP ? tmax(s[k], v[c][k]) : s[k] + v[c][k];
+          }
+        }
       }
       if (more) np1 = uniform(ptr[nr1]);
       block_combine<BLOCK, NV, MAXOP, T>(s, lds);
@@ -176,21 +234,14 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
       decltype(pre(0)) pr[kRowsPerLane];
 #pragma unroll
       for (int k = 0; k < kRowsPerLane; ++k) {
+        // rows past the tile load its last row (valid) and are not used
         const int row = (int)threadIdx.x + k * BLOCK;
-        a[k] = b[k] = 0;
-        pr[k] = decltype(pre(0)){};
-        if (row < rows) {
-          a[k] = ptr[r0 + row] - p0;
-          b[k] = ptr[r0 + row + 1] - p0;
-          pr[k] = pre(r0 + row);
-        }
+        const int rc = r0 + (row < rows ? row : rows - 1);
+        a[k] = ptr[rc] - p0;
+        b[k] = ptr[rc + 1] - p0;
+        pr[k] = pre(rc);
       }
-      for (int q = (int)threadIdx.x; q < nz; q += BLOCK) {
-        T v[NV];
-        prod(p0 + q, idx[p0 + q], v);
-#pragma unroll
-        for (int k = 0; k < NV; ++k) lds[k * TNNZ + q] = v[k];
-      }
+      stage_products<BLOCK, TNNZ, NV, T>(idx, p0, nz, prod, lds);
       __syncthreads();
       if (more) np1 = uniform(ptr[nr1]);
 #pragma unroll
@@ -226,20 +277,12 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
       tpr = tpr >= 64 ? 64 : (tpr >= 32 ? 32 : (tpr >= 16 ? 16 : (tpr >= 8 ? 8 : (tpr >= 4 ? 4 : (tpr >= 2 ? 2 : 1)))));
       const int grp = (int)threadIdx.x / tpr, ln = (int)threadIdx.x & (tpr - 1);
       const bool active = grp < rows;
-      // row extent and epilogue operands: issued before the product loads
-      int a = 0, b = 0;
-      decltype(pre(0)) pr{};
-      if (active) {
-        a = ptr[r0 + grp] - p0;
-        b = ptr[r0 + grp + 1] - p0;
-        if (ln == 0) pr = pre(r0 + grp);
-      }
-      for (int q = (int)threadIdx.x; q < nz; q += BLOCK) {
-        T v[NV];
-        prod(p0 + q, idx[p0 + q], v);
-#pragma unroll
-        for (int k = 0; k < NV; ++k) lds[k * TNNZ + q] = v[k];
-      }
+      // row extent and epilogue operands, issued before the product loads and
+      // without a branch (idle groups load the tile's last row and ignore it)
+      const int rc = r0 + (active ? grp : rows - 1);
+      const int a = ptr[rc] - p0, b = ptr[rc + 1] - p0;
+      const auto pr = pre(rc);
+      stage_products<BLOCK, TNNZ, NV, T>(idx, p0, nz, prod, lds);
       __syncthreads();
       if (more) np1 = uniform(ptr[nr1]);
       T s[NV];
