@@ -126,7 +126,7 @@ constexpr int kRowsPerLane = 4;
 // chunk, then its products (values and gathered entries), then the LDS
 // stores.  A plain loop over the nonzeros would wait for each index before its
 // gather, two dependent memory round trips per nonzero.
-constexpr int kChunk = 8;
+constexpr int kChunk = 4;
 
 // Stage the products of nonzeros [p0, p0 + nz) of a tile (nz <= TNNZ) into
 // LDS slot q = nonzero - p0.  Lanes past the end load the tile's last nonzero
