@@ -91,11 +91,12 @@ __device__ __forceinline__ void transition_in_lds(const LpDev<T>& L, int ph, con
 }
 
 // One workgroup per LP (blockIdx.x): up to `max_phases` phases of the LP in a
-// row.  Phase k+1 reads what phase k wrote to global memory; the workgroup
+// row.  Register use is capped for 4 waves per SIMD (a few spills, measured
+// faster at the batch's p50 than 3 waves without them).  Phase k+1 reads what phase k wrote to global memory; the workgroup
 // barrier between them orders those accesses (all of a workgroup's waves share
 // one CU).  The loop is bounded, and the state goes back to HBM at the end.
 template <class T, int BLOCK, int TNNZ>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8)))
     pdlp_run_kernel(const LpDev<T>* __restrict__ lps, const StepCtl* __restrict__ ctl,
                     unsigned* done, int max_phases) {
   __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];

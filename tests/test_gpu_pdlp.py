@@ -143,3 +143,22 @@ def test_fp32_refined_maximisation_and_free_variables():
         s = solve_pdlp(p, PDLPParams(device="gpu", precision=32, eps_abs=1e-6, eps_rel=1e-6))
         assert s.status == Status.OPTIMAL and abs(s.objective - expect) < 1e-8 * max(1, expect)
         np.testing.assert_allclose(s.reduced_costs, p.c - p.A.rmatvec(s.y), rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_skewed_lp_over_many_workgroups_matches_closed_form(precision):
+    """A 20k x 100k LP with Zipf column degrees (the 200k x 1M construction at a
+    tenth of the size): every phase over W workgroups; FP64 and FP32 +
+    refinement reach the closed-form optimum."""
+    from sovlp.workloads import skewed_lp
+    print(f"[gpu] skewed 20k x 100k, precision {precision}", flush=True)
+    ko = skewed_lp(1, 20_000, 100_000, 800_000)
+    par = PDLPParams(device="gpu", eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6,
+                     precision=precision, team="grid")
+    s = solve_pdlp(ko.problem, par)
+    assert s.status == Status.OPTIMAL
+    assert s.info.workgroups > 1
+    tol = 1e-8 if precision == 32 else 1e-6
+    assert abs(s.objective - ko.f_star) <= tol * max(1.0, abs(ko.f_star))
+    rv, bv, _ = ko.problem.violation(s.x)
+    assert rv <= 1e-6 and bv <= 1e-9
