@@ -131,15 +131,7 @@ constexpr int kChunk = 4;
 // Stage the products of nonzeros [p0, p0 + nz) of a tile (nz <= TNNZ) into
 // LDS slot q = nonzero - p0.  Lanes past the end load the tile's last nonzero
 // (always valid) and discard it.
-// Column index load; NT: non-temporal (the index stream bypasses the caches
-// that hold the gathered vector).
-template <bool NT>
-__device__ __forceinline__ int32_t load_idx(const int32_t* __restrict__ idx, int q) {
-  if constexpr (NT) return __builtin_nontemporal_load(idx + q);
-  else return idx[q];
-}
-
-template <int BLOCK, int TNNZ, int NV, class T, bool NT, class Prod>
+template <int BLOCK, int TNNZ, int NV, class T, class Prod>
 __device__ __forceinline__ void stage_products(const int32_t* __restrict__ idx, int p0, int nz,
                                                Prod& prod, T* lds) {
   constexpr int PER = (TNNZ + BLOCK - 1) / BLOCK;
@@ -152,7 +144,7 @@ __device__ __forceinline__ void stage_products(const int32_t* __restrict__ idx, 
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
       const int q = (int)threadIdx.x + (c + k) * BLOCK;
-      jj[k] = load_idx<NT>(idx, p0 + (q < nz ? q : nz - 1));
+      jj[k] = idx[p0 + (q < nz ? q : nz - 1)];
     }
     T v[CH][NV];
 #pragma unroll
@@ -186,8 +178,7 @@ __device__ __forceinline__ void stage_products(const int32_t* __restrict__ idx, 
 // runs the tiles one after another, so the number of dependent memory
 // latencies per tile, not bandwidth, sets the time of a small pass.
 // ------------------------------------------------------------------------
-template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, bool NT = false, class Prod,
-          class Pre, class Epi>
+template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Prod, class Pre, class Epi>
 __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
                                              const int32_t* __restrict__ idx,
                                              const int32_t* __restrict__ tiles, int t0, int t1,
@@ -215,7 +206,7 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
 #pragma unroll
         for (int c = 0; c < kChunk; ++c) {
           const int q = base + (int)threadIdx.x + c * BLOCK;
-          jj[c] = load_idx<NT>(idx, q < p1 ? q : p1 - 1);
+          jj[c] = idx[q < p1 ? q : p1 - 1];
         }
         T v[kChunk][NV];
 #pragma unroll
@@ -250,7 +241,7 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
         b[k] = ptr[rc + 1] - p0;
         pr[k] = pre(rc);
       }
-      stage_products<BLOCK, TNNZ, NV, T, NT>(idx, p0, nz, prod, lds);
+      stage_products<BLOCK, TNNZ, NV, T>(idx, p0, nz, prod, lds);
       __syncthreads();
       if (more) np1 = uniform(ptr[nr1]);
 #pragma unroll
@@ -291,7 +282,7 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
       const int rc = r0 + (active ? grp : rows - 1);
       const int a = ptr[rc] - p0, b = ptr[rc + 1] - p0;
       const auto pr = pre(rc);
-      stage_products<BLOCK, TNNZ, NV, T, NT>(idx, p0, nz, prod, lds);
+      stage_products<BLOCK, TNNZ, NV, T>(idx, p0, nz, prod, lds);
       __syncthreads();
       if (more) np1 = uniform(ptr[nr1]);
       T s[NV];
@@ -321,15 +312,14 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
 }
 
 // The tile pass without epilogue prefetch: epi(row, s[NV]).
-template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, bool NT = false, class Prod,
-          class Epi>
+template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Prod, class Epi>
 __device__ __forceinline__ void tile_pass(const int32_t* __restrict__ ptr,
                                           const int32_t* __restrict__ idx,
                                           const int32_t* __restrict__ tiles, int t0, int t1,
                                           Prod& prod, Epi& epi, T* lds) {
   auto pre = [](int) { return 0; };
   auto epi3 = [&](int i, const T (&s)[NV], int) { epi(i, s); };
-  tile_pass_pf<BLOCK, TNNZ, NV, MAXOP, T, NT>(ptr, idx, tiles, t0, t1, prod, pre, epi3, lds);
+  tile_pass_pf<BLOCK, TNNZ, NV, MAXOP, T>(ptr, idx, tiles, t0, t1, prod, pre, epi3, lds);
 }
 
 // Visit every nonzero of a tile range with its row: fn(q, row, col).

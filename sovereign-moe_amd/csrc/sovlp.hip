@@ -172,10 +172,9 @@ __global__ void __launch_bounds__(BLOCK)
                       T* __restrict__ y) {
   __shared__ __align__(16) T lds[TNNZ];
   const int t0 = part[blockIdx.x], t1 = part[blockIdx.x + 1];
-  // the matrix streams through once (non-temporal loads); the caches keep x
-  auto prod = [&](int q, int j, T (&v)[1]) { v[0] = __builtin_nontemporal_load(val + q) * x[j]; };
+  auto prod = [&](int q, int j, T (&v)[1]) { v[0] = val[q] * x[j]; };
   auto epi = [&](int r, const T (&s)[1]) { y[r] = s[0]; };
-  tile_pass<BLOCK, TNNZ, 1, false, T, true>(ptr, idx, tiles, t0, t1, prod, epi, lds);
+  tile_pass<BLOCK, TNNZ, 1, false, T>(ptr, idx, tiles, t0, t1, prod, epi, lds);
 }
 
 // ---------------------------------------------------------------- workspace
