@@ -118,6 +118,24 @@ int sovlp_spmv_tiled_f32(int rows, const int32_t *ptr32, const int32_t *idx, con
                          const int32_t *tiles, int ntiles, const int32_t *part, int grid,
                          const float *x, float *y, sovlp_stream_t stream);
 
+/* y = A x with A stored as `panels` column panels (XCD-aware): ptrs holds
+ * panels x (rows+1) int32 pointers into the concatenated idx / val (panel p
+ * holds the entries of the columns assigned to it), tiles the concatenated
+ * tile plans of the panels (panel p's start at tile_off[p]), parts panels x
+ * (groups+1) tile-range splits (sovlp_partition_tiles per panel).  Workgroup b
+ * works on panel b % panels, so with 8 panels each XCD gathers from one slice
+ * of x.  partial: panels x rows scratch; y[r] = the panels' row sums added in
+ * panel order.  Two launches.  No reference counterpart (backend.py:118-152 is
+ * row-per-warp only). */
+int sovlp_spmv_panels_f64(int panels, int rows, const int32_t *ptrs, const int32_t *idx,
+                          const double *val, const int32_t *tiles, const int32_t *tile_off,
+                          const int32_t *parts, int groups, const double *x, double *partial,
+                          double *y, sovlp_stream_t stream);
+int sovlp_spmv_panels_f32(int panels, int rows, const int32_t *ptrs, const int32_t *idx,
+                          const float *val, const int32_t *tiles, const int32_t *tile_off,
+                          const int32_t *parts, int groups, const float *x, float *partial,
+                          float *y, sovlp_stream_t stream);
+
 /* ---- PDLP engine: src/sovopt/lp/pdlp.py:238-494 ---------------------- */
 
 /* PDLPParams, pdlp.py:79-116 (same fields and defaults), then the

@@ -177,6 +177,41 @@ __global__ void __launch_bounds__(BLOCK)
   tile_pass<BLOCK, TNNZ, 1, false, T>(ptr, idx, tiles, t0, t1, prod, epi, lds);
 }
 
+// ------------------------------------------------------- column-panel SpMV
+// y = A x with A split into P column panels, each its own CSR (pointers
+// ptrs[p * (rows + 1) + r] index the concatenated idx / val).  Workgroup b
+// works on panel b % P, so with workgroups dispatched round-robin over the
+// XCDs every XCD gathers from one slice of x, which its L2 holds; each panel
+// writes its row sums to partial[p * rows + r].
+template <class T, int BLOCK, int TNNZ>
+__global__ void __launch_bounds__(BLOCK)
+    spmv_panel_kernel(int panels, int rows, const int32_t* __restrict__ ptrs,
+                      const int32_t* __restrict__ idx, const T* __restrict__ val,
+                      const int32_t* __restrict__ tiles, const int32_t* __restrict__ tile_off,
+                      const int32_t* __restrict__ parts, int groups, const T* __restrict__ x,
+                      T* __restrict__ partial) {
+  __shared__ __align__(16) T lds[TNNZ];
+  const int p = (int)blockIdx.x % panels, g = (int)blockIdx.x / panels;
+  const int32_t* pp = parts + (size_t)p * (groups + 1);
+  const int t0 = pp[g], t1 = pp[g + 1];
+  T* out = partial + (size_t)p * rows;
+  auto prod = [&](int q, int j, T (&v)[1]) { v[0] = val[q] * x[j]; };
+  auto epi = [&](int r, const T (&s)[1]) { out[r] = s[0]; };
+  tile_pass<BLOCK, TNNZ, 1, false, T>(ptrs + (size_t)p * (rows + 1), idx, tiles + tile_off[p],
+                                      t0, t1, prod, epi, lds);
+}
+
+// y[r] = sum of the panels' partial row sums, in panel order
+template <class T>
+__global__ void panel_sum_kernel(int panels, int rows, const T* __restrict__ partial,
+                                 T* __restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  T s = partial[r];
+  for (int p = 1; p < panels; ++p) s += partial[(size_t)p * rows + r];
+  y[r] = s;
+}
+
 // ---------------------------------------------------------------- workspace
 struct Carver {
   char* base;
@@ -403,7 +438,38 @@ static int spmv_tiled(int rows, const int32_t* ptr32, const int32_t* idx, const 
   return 0;
 }
 
+template <class T, int TNNZ>
+static int spmv_panels(int panels, int rows, const int32_t* ptrs, const int32_t* idx,
+                       const T* val, const int32_t* tiles, const int32_t* tile_off,
+                       const int32_t* parts, int groups, const T* x, T* partial, T* y,
+                       hipStream_t st) {
+  if (panels < 1 || rows < 0 || groups < 1) return fail(1, "bad panel SpMV arguments");
+  if ((long long)panels * groups > (1LL << 30)) return fail(1, "too many workgroups");
+  if (rows == 0) return 0;
+  spmv_panel_kernel<T, kBlock, TNNZ><<<panels * groups, kBlock, 0, st>>>(
+      panels, rows, ptrs, idx, val, tiles, tile_off, parts, groups, x, partial);
+  HIPCHK(hipGetLastError());
+  panel_sum_kernel<T><<<(rows + 255) / 256, 256, 0, st>>>(panels, rows, partial, y);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 extern "C" {
+
+int sovlp_spmv_panels_f64(int panels, int rows, const int32_t* ptrs, const int32_t* idx,
+                          const double* val, const int32_t* tiles, const int32_t* tile_off,
+                          const int32_t* parts, int groups, const double* x, double* partial,
+                          double* y, sovlp_stream_t stream) {
+  return spmv_panels<double, kTnnz64>(panels, rows, ptrs, idx, val, tiles, tile_off, parts,
+                                      groups, x, partial, y, S(stream));
+}
+int sovlp_spmv_panels_f32(int panels, int rows, const int32_t* ptrs, const int32_t* idx,
+                          const float* val, const int32_t* tiles, const int32_t* tile_off,
+                          const int32_t* parts, int groups, const float* x, float* partial,
+                          float* y, sovlp_stream_t stream) {
+  return spmv_panels<float, kTnnz32>(panels, rows, ptrs, idx, val, tiles, tile_off, parts,
+                                     groups, x, partial, y, S(stream));
+}
 
 int sovlp_spmv_tiled_f64(int rows, const int32_t* ptr32, const int32_t* idx, const double* val,
                          const int32_t* tiles, int ntiles, const int32_t* part, int grid,

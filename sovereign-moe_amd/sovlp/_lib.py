@@ -74,6 +74,10 @@ _SIGS = {
                                       C.POINTER(C.c_int)]),
     "sovlp_spmv_tiled_f64": (C.c_int, [C.c_int, P, P, P, P, C.c_int, P, C.c_int, P, P, P]),
     "sovlp_spmv_tiled_f32": (C.c_int, [C.c_int, P, P, P, P, C.c_int, P, C.c_int, P, P, P]),
+    "sovlp_spmv_panels_f64": (C.c_int, [C.c_int, C.c_int, P, P, P, P, P, P, C.c_int, P, P, P,
+                                        P]),
+    "sovlp_spmv_panels_f32": (C.c_int, [C.c_int, C.c_int, P, P, P, P, P, P, C.c_int, P, P, P,
+                                        P]),
     "sovlp_pdlp_workspace_bytes": (C.c_int, [C.POINTER(sovlp_lp), C.c_int, I64, C.POINTER(I64)]),
     "sovlp_pdlp_create": (C.c_int, [C.POINTER(sovlp_lp), C.POINTER(sovlp_params), C.c_int, P,
                                     I64, I64, C.c_int, P, C.POINTER(P)]),

@@ -124,3 +124,21 @@ def test_backend_refuses_cpu_and_missing_gpu():
     if not backend.gpu_available():
         with pytest.raises(RuntimeError):
             backend.get_backend("gpu")
+
+
+def test_column_panel_split_reproduces_the_matrix():
+    from sovlp.spmv_bench import split_panels
+    rng = np.random.default_rng(4)
+    A = SparseMatrix.from_triplets(rng.integers(0, 40, 600), rng.integers(0, 97, 600),
+                                   rng.standard_normal(600), 40, 97)
+    for panels in (1, 3, 8):
+        ptrs, idx, val = split_panels(A.rp, A.ri, A.rx, A.n, panels)
+        w = -(-A.n // panels)
+        D = np.zeros((A.m, A.n))
+        for p in range(panels):
+            for r in range(A.m):
+                cols = idx[ptrs[p, r]:ptrs[p, r + 1]]
+                assert np.all((cols // w) == p) and np.all(np.diff(cols) > 0)
+                D[r, cols] += val[ptrs[p, r]:ptrs[p, r + 1]]
+        np.testing.assert_array_equal(D, A.to_dense())
+        assert ptrs[-1, -1] == A.nnz
