@@ -70,19 +70,20 @@ __device__ __forceinline__ void take_transition(const LpDev<T>& L, const LpState
   *L.st = ss;
 }
 
-// Thread 0: apply the transition of phase `ph` to the state held in LDS.
+// Thread 0: apply the transition of phase `ph` to the state held in LDS;
+// `pc` are the launch's phase counters (LDS) and `P` the parameters (LDS).
 template <class T>
 __device__ __forceinline__ void transition_in_lds(const LpDev<T>& L, int ph, const double* t,
-                                                  const StepCtl* ctl, unsigned* done,
-                                                  LpState& ss) {
+                                                  const StepCtl* ctl, const Params& P,
+                                                  unsigned* done, int* pc, LpState& ss) {
   if (ph == PH_INIT) {
     ss.tick0 = (long long)wall_clock64();
     ss.ticks_left = ctl->ticks;
   }
-  L.pcount[ph] += 1;
+  pc[ph] += 1;
   if ((ph == PH_B_ADAPT || ph == PH_B_FUSED) && ss.next_mode != NEXT_NONE)
-    L.pcount[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
-  Transition tr{ctl->P, L.offset0, L.flip};
+    pc[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
+  Transition tr{P, L.offset0, L.flip};
   ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap, L.rhist);
   if (ss.phase == PH_DONE) {
     ss.ticks = (long long)wall_clock64() - ss.tick0;
@@ -102,8 +103,14 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
   __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   __shared__ LpState ss;
+  __shared__ Params sp;                  // the parameters, read once per launch
+  __shared__ int pc[kPhaseSlots];        // phase counts of this launch
   const LpDev<T>& L = lps[blockIdx.x];
-  if (threadIdx.x == 0) ss = *L.st;
+  if (threadIdx.x == 0) {
+    ss = *L.st;
+    sp = ctl->P;
+  }
+  if (threadIdx.x < kPhaseSlots) pc[threadIdx.x] = 0;
   __syncthreads();
   if (uniform(ss.phase) == PH_DONE) return;
   for (int it = 0; it < max_phases; ++it) {
@@ -121,15 +128,16 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
     if (reduce) reduce_phase_to_lds<BLOCK>(ph, part, red);
     if (threadIdx.x == 0) {
       if (ph == PH_A_FUSED) {
-        L.pcount[PH_A_FUSED] += 1;
+        pc[PH_A_FUSED] += 1;
         ss.phase = PH_B_FUSED;
       } else {
-        transition_in_lds(L, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
+        transition_in_lds(L, ph, red + (BLOCK / kWave) * kMaxRed, ctl, sp, done, pc, ss);
       }
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) *L.st = ss;
+  if (threadIdx.x < kPhaseSlots && pc[threadIdx.x] != 0) L.pcount[threadIdx.x] += pc[threadIdx.x];
 }
 
 // W workgroups share one LP's phase; each writes its partial sums.
