@@ -52,9 +52,10 @@ class PDLPParams:
     precision: int = 64
     team: str = "auto"
     chunk_iters: int = 8192
-    # iterative refinement of the FP32 engine (None: 8 rounds at precision
+    # iterative refinement of the FP32 engine (None: 5 rounds at precision
     # 32, none at 64; each round to relative KKT max(eps_rel, 1e-4), which
-    # FP32 iterates reach; the rounds together reach eps and feas_cap)
+    # FP32 iterates reach; the rounds together reach eps and feas_cap).  On
+    # 4096 uneven LPs, rounds 6-8 finished no further LP.
     refine_rounds: int | None = None
     refine_eps: float | None = None
     refine_tol: float = 1e-9
@@ -63,7 +64,7 @@ class PDLPParams:
         prec = int(self.precision if precision is None else precision)
         rounds = self.refine_rounds
         if rounds is None:
-            rounds = 8 if prec == 32 else 0
+            rounds = 5 if prec == 32 else 0
         eps = self.refine_eps if self.refine_eps is not None else max(self.eps_rel, 1e-4)
         return int(rounds), float(eps)
 
