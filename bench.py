@@ -58,6 +58,9 @@ def parse():
                     help="32: FP32 iterates + FP64 reductions + iterative refinement (default)")
     ap.add_argument("--block", type=int, default=256, choices=(256, 64))
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--refine-stall", type=int, default=0,
+                    help="FP32: iterations without a halving of a round's KKT ratio that end the "
+                         "round (0: the engine's 4096); shorter trades finished LPs for time")
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="LPs solved one at a time for the FP64 single-LP rate (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -167,7 +170,8 @@ def main():
     problems = [k.problem for k in kos]
     log(f"generated {a.lps} LPs in {time.perf_counter() - t:.1f} s "
         f"(sum nnz {sum(p.A.nnz for p in problems)})")
-    par = PDLPParams(eps_abs=a.tol, eps_rel=a.tol, feas_cap=a.tol, device="gpu")
+    par = PDLPParams(eps_abs=a.tol, eps_rel=a.tol, feas_cap=a.tol, device="gpu",
+                     refine_stall=a.refine_stall)
     tuner = None
     if a.tuning != "none":
         from sovlp.lp.autotune import Autotuner, packaged_table

@@ -146,6 +146,8 @@ int sovlp_spmv_panels_f32(int panels, int rows, const int32_t *ptrs, const int32
  *                rl <= A x^ <= ru and lo <= x^ <= hi shifted by the FP64
  *                accumulated X (and A X) and scaled by a power of two Delta;
  *                X += x^ / Delta.  The dual carries over between rounds.
+ * refine_stall   a round whose KKT ratio has not halved for this many
+ *                iterations ends (0: 4096), and every round ends after 8x that
  * refine_eps     relative tolerance (eps_abs = eps_rel) of every round
  * refine_tol     the loop also stops once the primal residual of X is
  *                <= refine_tol (1 + ||finite row bounds||_inf) and the duality
@@ -157,7 +159,7 @@ typedef struct {
   double time_limit;
   int32_t check_every, stepsize_every;
   double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
-  int32_t refine_rounds, refine_pad;
+  int32_t refine_rounds, refine_stall;
   double refine_eps, refine_tol;
 } sovlp_params;
 

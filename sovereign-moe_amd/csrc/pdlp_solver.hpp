@@ -51,7 +51,9 @@ struct Params {              // same layout as sovlp_params in include/sovlp.h
   double time_limit;
   int check_every, stepsize_every;
   double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
-  int refine_rounds, refine_pad;   // 0: no refinement (FP64 engines always 0)
+  int refine_rounds;         // 0: no refinement (FP64 engines always 0)
+  int refine_stall;          // a round whose KKT ratio has not halved for this many
+                             // iterations stops (0: 4096); it stops at 8x this in any case
   double refine_eps;         // relative KKT tolerance of each refinement round
   double refine_tol;         // primal residual target: refine_tol (1 + ||b||_inf)
 };
@@ -1047,9 +1049,9 @@ struct Transition {
           gscale = fmax(gscale, s.ref_scale * s.ref_scale * s.gap_last);
         const double gtol = ea + er * gscale;
         if (refining()) {
-          // a round stalls when its KKT ratio has not halved for 4096
-          // iterations (FP32 iterates can stall short of the round's
-          // tolerance), or after 32768.  If the last X met _converged, X is
+          // a round stalls when its KKT ratio has not halved for
+          // refine_stall iterations (FP32 iterates can stall short of the
+          // round's tolerance), or after 8x that.  If the best X met _converged, it is
           // reported and the round dropped; otherwise the solve refines from
           // the round's current candidate (a round that then cuts the gap by
           // less than 10x makes the next one zoom out)
@@ -1058,7 +1060,8 @@ struct Transition {
             s.kkt_best = ratio;
             s.k_best = s.k;
           }
-          if (ratio > 1.0 && (s.k - s.k_best >= 4096 || s.k >= 32768)) {
+          const long long stall = P.refine_stall > 0 ? P.refine_stall : 4096;
+          if (ratio > 1.0 && (s.k - s.k_best >= stall || s.k >= 8 * stall)) {
             if (s.round > 0 && s.best_q <= 1.0) {
               // the best X evaluated so far met _converged: report it
               s.absorbed = 1;

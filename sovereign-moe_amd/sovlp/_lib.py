@@ -29,7 +29,7 @@ class sovlp_params(C.Structure):
     _fields_ = [("eps_abs", F64), ("eps_rel", F64), ("feas_cap", F64), ("max_iter", I64),
                 ("time_limit", F64), ("check_every", I32), ("stepsize_every", I32),
                 ("beta_sufficient", F64), ("beta_necessary", F64), ("beta_artificial", F64),
-                ("primal_weight_smoothing", F64), ("refine_rounds", I32), ("refine_pad", I32),
+                ("primal_weight_smoothing", F64), ("refine_rounds", I32), ("refine_stall", I32),
                 ("refine_eps", F64), ("refine_tol", F64)]
 
 

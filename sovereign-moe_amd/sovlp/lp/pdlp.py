@@ -59,6 +59,7 @@ class PDLPParams:
     refine_rounds: int | None = None
     refine_eps: float | None = None
     refine_tol: float = 1e-9
+    refine_stall: int = 0          # iterations without a halving that end a round (0: 4096)
 
     def resolved_refine(self, precision: int | None = None) -> tuple[int, float]:
         prec = int(self.precision if precision is None else precision)
@@ -74,7 +75,8 @@ class PDLPParams:
                                  float(self.time_limit), int(self.check_every),
                                  int(self.stepsize_every), self.beta_sufficient,
                                  self.beta_necessary, self.beta_artificial,
-                                 self.primal_weight_smoothing, rounds, 0, eps,
+                                 self.primal_weight_smoothing, rounds,
+                                 int(self.refine_stall), eps,
                                  float(self.refine_tol))
 
 
