@@ -8,13 +8,17 @@ whole batch once, from scratch (scaling, power iteration, PDHG to termination
 under the reference's _converged test with eps_abs = eps_rel = feas_cap =
 1e-6), with the inputs resident in HBM.  Default arithmetic: FP32 iterates
 and matrix, FP64 reductions, and an outer iterative refinement (FP64 X, Y)
-that takes each objective to within ~1e-8 of the FP64 optimum.  `value`
+that takes each objective to within ~1e-8 of the FP64 optimum, with rounds
+cut short once their KKT ratio stops halving for 1024 iterations.  `value`
 counts only the LPs that end OPTIMAL (the criterion met), per second of the
 timed steps.
 
 Beside the headline number the JSON line carries:
   plain_fp64    the same LPs one at a time through the single-LP engine in
                 FP64 (inputs resident, a sample), and the ratio
+  engine_default_stalls  the batch again with the engine's default stall
+                lengths (4096 before a refined point meets the criterion):
+                more LPs finish, in more time
   roofline      algorithmic bytes of the step kernel (phase counts x the byte
                 model of each phase) / its average launch time (HIP events on
                 the launch stream), against 8 TB/s
@@ -59,8 +63,14 @@ def parse():
     ap.add_argument("--block", type=int, default=256, choices=(256, 64))
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--refine-stall", type=int, default=0,
-                    help="FP32: iterations without a halving of a round's KKT ratio that end the "
-                         "round (0: the engine's 4096); shorter trades finished LPs for time")
+                    help="FP32: iterations without a halving of a round's KKT ratio that end a "
+                         "round polishing a point that met the criterion (0: the engine's 1024)")
+    ap.add_argument("--refine-stall-open", type=int, default=1024,
+                    help="FP32: the same for rounds before any refined point met the criterion "
+                         "(0: the engine's 4096, which finishes more LPs in twice the time)")
+    ap.add_argument("--robust-sample", type=int, default=1,
+                    help="FP32: steps also run with the engine's default stall lengths, "
+                         "reported beside the headline (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="LPs solved one at a time for the FP64 single-LP rate (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -76,7 +86,43 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int, tuned: bool):
+def refinement_config(par, precision: int):
+    """Refinement settings the engine ran with (None at FP64)."""
+    rounds, eps = par.resolved_refine(precision)
+    if rounds == 0:
+        return None
+    return {"rounds": rounds, "round_eps": eps, "stall": par.refine_stall or 1024,
+            "stall_open": par.refine_stall_open or 4096, "target": par.refine_tol}
+
+
+def robust_rate(problems, par, tuner, a) -> dict:
+    """The same batch with the engine's default stall lengths (rounds before a
+    refined point meets the criterion may run 4x longer): more LPs finish, in
+    more time.  One untimed step, then ``a.robust_sample`` timed ones."""
+    import dataclasses
+    import torch
+    from sovlp.lp.batch import BatchEngine
+    p2 = dataclasses.replace(par, refine_stall_open=0)
+    eng = BatchEngine(problems, p2, precision=a.precision, block=a.block, tuner=tuner)
+    try:
+        eng.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.robust_sample):
+            eng.run()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        sols, _lat = eng.solutions(keep_outputs=False)
+    finally:
+        eng.close()
+    opt = sum(1 for s in sols if int(s.status) == 0)
+    return {"value": opt * a.robust_sample / wall, "unit": "LPs/s", "optimal": opt,
+            "ms_per_step": wall / a.robust_sample * 1e3,
+            "refinement": refinement_config(p2, a.precision)}
+
+
+def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int, tuned: bool,
+                     refine: dict | None):
     """HBM bytes per launch of ``kernel`` on this workload from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's reading;
     see profiles/), or None when they were taken on another configuration."""
@@ -88,6 +134,9 @@ def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: in
     want = f"{lps} uneven LPs, seeds {seed}..{seed + lps - 1}, " \
            f"{'FP32 + refinement' if precision == 32 else 'FP64'}, block {block}" \
            f"{', autotuned' if tuned else ''}"
+    if refine:
+        want += f", {refine['rounds']} rounds at {refine['round_eps']:g}, stall " \
+                f"{refine['stall_open']}/{refine['stall']}"
     if d.get("kernel") != kernel or d.get("workload") != want:
         return None
     return float(d["traffic_bytes_per_launch"])
@@ -171,7 +220,7 @@ def main():
     log(f"generated {a.lps} LPs in {time.perf_counter() - t:.1f} s "
         f"(sum nnz {sum(p.A.nnz for p in problems)})")
     par = PDLPParams(eps_abs=a.tol, eps_rel=a.tol, feas_cap=a.tol, device="gpu",
-                     refine_stall=a.refine_stall)
+                     refine_stall=a.refine_stall, refine_stall_open=a.refine_stall_open)
     tuner = None
     if a.tuning != "none":
         from sovlp.lp.autotune import Autotuner, packaged_table
@@ -268,7 +317,8 @@ def main():
                                f"seeds {a.seed}..{a.seed + a.lps - 1}",
                    "lps": a.lps, "tolerance": a.tol, "precision": a.precision,
                    "block": a.block, "parallelism": "one workgroup per LP",
-                   "kernel_configs": "autotuned" if tuner is not None else f"block {a.block}"},
+                   "kernel_configs": "autotuned" if tuner is not None else f"block {a.block}",
+                   "refinement": refinement_config(par, a.precision)},
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "optimal": opt,
@@ -282,10 +332,15 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(kernel, a.lps, a.seed, a.precision, a.block,
-                                                 tuner is not None),
+                                                 tuner is not None,
+                                                 refinement_config(par, a.precision)),
                      "algorithmic_bytes_per_launch": step_bytes / graphs[-1],
                      "avg_launch_us": launch_ms * 1e3, "launches": launches},
     }
+    if a.precision == 32 and a.robust_sample > 0 and a.refine_stall_open != 0:
+        rb = robust_rate(problems, par, tuner, a)
+        log(f"default stall lengths: {rb}")
+        out["engine_default_stalls"] = rb
     if a.plain_sample > 0:
         t = time.perf_counter()
         pl = plain_fp64_rate(problems, a.tol, min(a.plain_sample, a.lps))

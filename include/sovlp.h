@@ -146,13 +146,17 @@ int sovlp_spmv_panels_f32(int panels, int rows, const int32_t *ptrs, const int32
  *                rl <= A x^ <= ru and lo <= x^ <= hi shifted by the FP64
  *                accumulated X (and A X) and scaled by a power of two Delta;
  *                X += x^ / Delta.  The dual carries over between rounds.
- * refine_stall   a round whose KKT ratio has not halved for this many
- *                iterations ends (0: 4096), and every round ends after 8x that
+ * refine_stall   a round run after an evaluated X has met _converged (one
+ *                that only chases refine_tol) ends when its KKT ratio has not
+ *                halved for this many iterations (0: 1024), or after 8x that
+ * refine_stall_open  the same for the rounds before that (0: 4096); shorter
+ *                finishes fewer LPs in less time
  * refine_eps     relative tolerance (eps_abs = eps_rel) of every round
  * refine_tol     the loop also stops once the primal residual of X is
  *                <= refine_tol (1 + ||finite row bounds||_inf) and the duality
  *                gap of X, Y is <= refine_tol (1 + |pobj| + |dobj|), after
- *                _converged (pdlp.py:523-530) holds for X on the original. */
+ *                _converged (pdlp.py:523-530) holds for X on the original.
+ * reserved       0 */
 typedef struct {
   double eps_abs, eps_rel, feas_cap;
   int64_t max_iter;
@@ -161,6 +165,7 @@ typedef struct {
   double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
   int32_t refine_rounds, refine_stall;
   double refine_eps, refine_tol;
+  int32_t refine_stall_open, reserved;
 } sovlp_params;
 
 /* One LP in minimisation form (costs and offset already negated for a

@@ -53,9 +53,12 @@ struct Params {              // same layout as sovlp_params in include/sovlp.h
   double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
   int refine_rounds;         // 0: no refinement (FP64 engines always 0)
   int refine_stall;          // a round whose KKT ratio has not halved for this many
-                             // iterations stops (0: 4096); it stops at 8x this in any case
+                             // iterations stops (0: 1024); it stops at 8x this in any case
   double refine_eps;         // relative KKT tolerance of each refinement round
   double refine_tol;         // primal residual target: refine_tol (1 + ||b||_inf)
+  int refine_stall_open;     // refine_stall while no evaluated X has met _converged
+                             // (0: 4096)
+  int pad_;
 };
 
 constexpr int kMaxRounds = 16;   // refinement rounds recorded per LP
@@ -1060,7 +1063,11 @@ struct Transition {
             s.kkt_best = ratio;
             s.k_best = s.k;
           }
-          const long long stall = P.refine_stall > 0 ? P.refine_stall : 4096;
+          // rounds that only polish an X that already met _converged (for the
+          // objective target) get refine_stall; rounds before that refine_stall_open
+          const long long stall_c = P.refine_stall > 0 ? P.refine_stall : 1024;
+          const long long stall = s.best_q <= 1.0 ? stall_c
+                                : (P.refine_stall_open > 0 ? P.refine_stall_open : 4096);
           if (ratio > 1.0 && (s.k - s.k_best >= stall || s.k >= 8 * stall)) {
             if (s.round > 0 && s.best_q <= 1.0) {
               // the best X evaluated so far met _converged: report it

@@ -900,6 +900,8 @@ int check_params(const sovlp_params* p, int precision) {
     return fail(4, "iterative refinement is provided for precision 32 only");
   if (p->refine_rounds > 0 && !(p->refine_eps > 0.0 && p->refine_tol >= 0.0))
     return fail(1, "refine_eps must be > 0 and refine_tol >= 0");
+  if (p->refine_stall < 0 || p->refine_stall_open < 0)
+    return fail(1, "refine_stall and refine_stall_open must be >= 0");
   return 0;
 }
 

@@ -30,7 +30,8 @@ class sovlp_params(C.Structure):
                 ("time_limit", F64), ("check_every", I32), ("stepsize_every", I32),
                 ("beta_sufficient", F64), ("beta_necessary", F64), ("beta_artificial", F64),
                 ("primal_weight_smoothing", F64), ("refine_rounds", I32), ("refine_stall", I32),
-                ("refine_eps", F64), ("refine_tol", F64)]
+                ("refine_eps", F64), ("refine_tol", F64), ("refine_stall_open", I32),
+                ("reserved", I32)]
 
 
 class sovlp_lp(C.Structure):
