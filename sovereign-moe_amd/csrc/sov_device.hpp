@@ -124,8 +124,10 @@ constexpr int kRowsPerLane = 4;
 
 // Nonzeros per lane whose loads are issued together: the column indices of a
 // chunk, then its products (values and gathered entries), then the LDS
-// stores.  A plain loop over the nonzeros would wait for each index before its
-// gather, two dependent memory round trips per nonzero.
+// stores; the next chunk's indices are loaded with this chunk's products.  A
+// plain loop over the nonzeros would wait for each index before its gather,
+// two dependent memory round trips per nonzero.  (8 per lane spills registers
+// under the run kernel's 128-VGPR cap and is slower.)
 constexpr int kChunk = 4;
 
 // Stage the products of nonzeros [p0, p0 + nz) of a tile (nz <= TNNZ) into
@@ -137,14 +139,25 @@ __device__ __forceinline__ void stage_products(const int32_t* __restrict__ idx, 
   constexpr int PER = (TNNZ + BLOCK - 1) / BLOCK;
   constexpr int CH = PER < kChunk ? PER : kChunk;
   if (nz <= 0) return;
+  int jj[CH];
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    const int q = (int)threadIdx.x + k * BLOCK;
+    jj[k] = idx[p0 + (q < nz ? q : nz - 1)];
+  }
 #pragma unroll
   for (int c = 0; c < PER; c += CH) {
     if (c * BLOCK >= nz) break;          // nz is uniform: a scalar branch
-    int jj[CH];
+    // the next chunk's column indices load while this chunk's products do:
+    // one memory latency per chunk instead of two
+    int jn[CH];
+    const bool next = c + CH < PER && (c + CH) * BLOCK < nz;
+    if (next) {
 #pragma unroll
-    for (int k = 0; k < CH; ++k) {
-      const int q = (int)threadIdx.x + (c + k) * BLOCK;
-      jj[k] = idx[p0 + (q < nz ? q : nz - 1)];
+      for (int k = 0; k < CH; ++k) {
+        const int q = (int)threadIdx.x + (c + CH + k) * BLOCK;
+        jn[k] = idx[p0 + (q < nz ? q : nz - 1)];
+      }
     }
     T v[CH][NV];
 #pragma unroll
@@ -159,6 +172,10 @@ __device__ __forceinline__ void stage_products(const int32_t* __restrict__ idx, 
 #pragma unroll
         for (int u = 0; u < NV; ++u) lds[u * TNNZ + q] = v[k][u];
       }
+    }
+    if (next) {
+#pragma unroll
+      for (int k = 0; k < CH; ++k) jj[k] = jn[k];
     }
   }
 }
