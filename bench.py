@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="wall budget of the CPU baseline sample (0: skip)")
     ap.add_argument("--spmv", action="store_true", help="also time the SpMV of the 200k x 1M LP")
+    ap.add_argument("--certify", action="store_true",
+                    help="also time the certificate (bound, reduced costs, compensated row "
+                         "activities) of a point of the 200k x 1M LP, and the C oracle on it")
     ap.add_argument("--tuning", default="auto",
                     help="autotuner table (JSON) choosing each LP's kernel configuration; "
                          "'auto': the packaged MI355X table for the precision if present; "
@@ -119,6 +122,53 @@ def robust_rate(problems, par, tuner, a) -> dict:
     return {"value": opt * a.robust_sample / wall, "unit": "LPs/s", "optimal": opt,
             "ms_per_step": wall / a.robust_sample * 1e3,
             "refinement": refinement_config(p2, a.precision)}
+
+
+def certify_rate(reps: int = 20) -> dict:
+    """One certificate of a point (x inside the box, y random) of the 200k x 1M
+    skewed LP: device time per call (kernels, the 48-byte result copy and the
+    synchronisation), its algorithmic bytes, and the C oracle on the same call
+    (one host thread)."""
+    import numpy as np
+    import torch
+    from sovlp.lp.certify import DeviceCertifier
+    from sovlp.workloads import skewed_lp
+    t0 = time.perf_counter()
+    k = skewed_lp(seed=1)
+    p = k.problem
+    rng = np.random.default_rng(9)
+    x = rng.uniform(0.0, 1.0, p.n) * np.minimum(p.col_ub, 10.0)
+    y = rng.standard_normal(p.m)
+    log(f"certificate: 200k x 1M LP built in {time.perf_counter() - t0:.1f} s")
+    dc = DeviceCertifier(p)
+    xs = torch.from_numpy(x).cuda()
+    ys = torch.from_numpy(y).cuda()
+    for _ in range(3):
+        dc.certify(xs, ys)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        cert = dc.certify(xs, ys)
+    us = (time.perf_counter() - t) / reps * 1e6
+    m, n, nnz = p.m, p.n, p.A.nnz
+    # rows: CSR (12 B per nonzero + 8 B pointers), x gathers, rl ru y, yy act
+    # written; columns: CSC, yy gathers, c x lo hi, d written; terms: d lo hi
+    alg = 20 * nnz + 8 * (m + 1) + 40 * m + 20 * nnz + 8 * (n + 1) + 40 * n + 24 * n
+    out = {"lp": "200000 x 1000000, %d nonzeros" % nnz, "us_per_certificate": us,
+           "algorithmic_bytes": alg, "achieved_gbs": alg / (us * 1e-6) / 1e9,
+           "frac_of_hbm_peak": alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+           "bound": cert.bound, "perturbation": cert.perturbation}
+    try:
+        from oracle import oracle
+        t = time.perf_counter()
+        ref = oracle.certify(p, x, y)
+        cpu = time.perf_counter() - t
+        out["cpu_baseline"] = {"seconds_per_certificate": cpu, "cores": 1, "kind": "port",
+                               "bound_rel_diff": abs(ref.bound - cert.bound) /
+                               max(1.0, abs(ref.bound))}
+    except (ImportError, OSError) as e:
+        out["cpu_baseline"] = {"error": str(e)}
+    return out
 
 
 def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int, tuned: bool,
@@ -355,6 +405,9 @@ def main():
         from sovlp.spmv_bench import skewed_roofline
         res, gen = skewed_roofline()
         out["spmv"] = res
+    if a.certify:
+        out["certify"] = certify_rate()
+        log(f"certificate: {out['certify']}")
     print(json.dumps(out), flush=True)
 
 

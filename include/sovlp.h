@@ -55,6 +55,52 @@ int sovlp_spmv_csr_f64(int rows, const int64_t *rp, const int32_t *ri, const dou
 int sovlp_spmv_csr_f32(int rows, const int64_t *rp, const int32_t *ri, const float *rx,
                        const float *x, float *y, int lanes_per_row, sovlp_stream_t stream);
 
+/* out = b - A x to about double-double accuracy (A in CSR, int64 pointers):
+ * each row's dot product with Dekker's two_product and Knuth's two_sum in
+ * CSR order, bit for bit the reference's compensated_residual
+ * (src/sovopt/numerics/refine.py:48-84, 103-110).  out may not alias x or b.
+ * Rows of more than 128 nonzeros are listed in `workspace`
+ * (sovlp_residual_workspace_bytes) and summed by one workgroup each. */
+int sovlp_residual_workspace_bytes(int rows, int64_t *bytes);
+int sovlp_residual_f64(int rows, const int64_t *rp, const int32_t *ri, const double *rx,
+                       const double *x, const double *b, double *out, void *workspace,
+                       int64_t workspace_bytes, sovlp_stream_t stream);
+
+/* The certificate of a point x and dual vector y (textbook signs, d = c - A^T y)
+ * on an LP in the model's own sense:
+ *   bound          certified_bound (src/sovopt/mip/safebound.py:252-307):
+ *                  a lower bound on min c'x (upper bound on max c'x) valid
+ *                  for any y, offset included; -inf (+inf) when vacuous.
+ *                  Duals pointing at infinite row bounds are taken as 0;
+ *                  reduced costs pointing at infinite column bounds whose
+ *                  largest magnitude is <= dual_tol (1 + ||c||_inf) are
+ *                  absorbed (taken as 0), larger ones make the bound vacuous.
+ *                  Strict: exact-to-a-few-ulps sums, less 4 eps sum |terms|
+ *                  (safebound.py:125-143).
+ *   perturbation   that largest magnitude (0 when no reduced cost points at
+ *                  an infinite bound).
+ *   objective      c'x + offset, compensated (bench/verify.py:227).
+ *   col_violation  max(lo - x, x - hi, 0) (bench/verify.py:208-211).
+ *   row_violation  max(rl - Ax, Ax - ru, 0) with A x compensated
+ *                  (bench/verify.py:214-218).
+ *   vacuous        1 when the bound is infinite. */
+typedef struct {
+  double bound, perturbation, objective, col_violation, row_violation;
+  int32_t vacuous, reserved;
+} sovlp_certificate;
+
+int sovlp_certify_workspace_bytes(int m, int n, int64_t *bytes);
+/* A: CSR (rp m+1, ri, rx) and CSC (cp n+1, ci, cx), int64 pointers; every
+ * array a device pointer.  `d_out` (n, optional) receives c - A^T y in the
+ * minimisation sense (before absorption), `act_out` (m, optional) the
+ * compensated row activities.  Synchronises `stream`; `out` is host memory. */
+int sovlp_certify_f64(int m, int n, const int64_t *rp, const int32_t *ri, const double *rx,
+                      const int64_t *cp, const int32_t *ci, const double *cx, const double *c,
+                      const double *rl, const double *ru, const double *lo, const double *hi,
+                      double offset, int maximise, const double *x, const double *y,
+                      double dual_tol, void *workspace, int64_t workspace_bytes, double *d_out,
+                      double *act_out, sovlp_certificate *out, sovlp_stream_t stream);
+
 /* xnew = clamp(x - tau (c + aty), lo, hi).  Replaces Backend.primal_step,
  * backend.py:404-415 (kernel 204-215). */
 int sovlp_primal_step_f64(double *xnew, const double *x, const double *c, const double *aty,

@@ -6,6 +6,10 @@ TEST INFRASTRUCTURE ONLY: imported by ``tests/``, the smoke entry point and the
 ``solve(prob, ...)`` restates ``solve_pdlp`` (reference ``src/sovopt/lp/pdlp.py:238-494``)
 and ``scale(prob)`` restates ``scale_problem(..., "pdlp")``
 (``src/sovopt/numerics/scaling.py:374-478``); see ``pdlp_oracle.c`` for the line map.
+``residual`` and ``certify`` restate ``compensated_residual``
+(``src/sovopt/numerics/refine.py:48-110``) and ``certified_bound``
+(``src/sovopt/mip/safebound.py:93-143, 175-186, 252-307``) with the feasibility lines of
+``bench/verify.py:208-224``; see ``certify_oracle.c``.
 """
 
 from __future__ import annotations
@@ -41,8 +45,8 @@ class _Info(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH) or \
-            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "pdlp_oracle.c")):
+    src = max(os.path.getmtime(os.path.join(HERE, f)) for f in ("pdlp_oracle.c", "certify_oracle.c"))
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < src:
         subprocess.run(["make", "-C", HERE, "-B" if force else "liboracle_pdlp.so"],
                        check=True, capture_output=True)
     return LIB_PATH
@@ -63,12 +67,71 @@ def lib():
         L.oracle_scale.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P]
         L.oracle_spmv.restype = None
         L.oracle_spmv.argtypes = [C.c_int, P, P, P, P, P]
+        L.oracle_residual.restype = None
+        L.oracle_residual.argtypes = [C.c_int, P, P, P, P, P, P]
+        L.oracle_certify.restype = None
+        L.oracle_certify.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P,
+                                     C.c_double, C.c_int, P, P, C.c_double, P, P, P, P, P,
+                                     C.POINTER(_Certificate)]
         _lib = L
     return _lib
 
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+class _Certificate(C.Structure):
+    _fields_ = [("bound", C.c_double), ("perturbation", C.c_double), ("objective", C.c_double),
+                ("col_violation", C.c_double), ("row_violation", C.c_double),
+                ("vacuous", C.c_int), ("pad", C.c_int)]
+
+
+@dataclass
+class Certificate:
+    bound: float
+    perturbation: float
+    objective: float
+    col_violation: float
+    row_violation: float
+    vacuous: bool
+    d: np.ndarray           # c - A^T yy, minimisation sense
+    act: np.ndarray         # compensated row activities A x
+
+
+def residual(rp, ri, rx, x, b) -> np.ndarray:
+    """``b - A x``, compensated, as the reference's ``compensated_residual``."""
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    ri = np.ascontiguousarray(ri, dtype=np.int32)
+    rx = np.ascontiguousarray(rx, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty(rp.size - 1)
+    lib().oracle_residual(rp.size - 1, _p(rp), _p(ri), _p(rx), _p(x), _p(b), _p(out))
+    return out
+
+
+def certify(prob, x, y, dual_tol: float = 1e-6) -> Certificate:
+    """The reference's ``certified_bound(prob, x, y, dual_tol)`` and the feasibility
+    figures of its ``verify``."""
+    A = prob.A
+    m, n = A.m, A.n
+    arrs = [np.ascontiguousarray(a) for a in (A.rp, A.ri, A.rx, A.cp, A.ci, A.cx, prob.c,
+                                              prob.row_lb, prob.row_ub, prob.col_lb,
+                                              prob.col_ub)]
+    arrs[0] = arrs[0].astype(np.int64)
+    arrs[3] = arrs[3].astype(np.int64)
+    arrs[1] = arrs[1].astype(np.int32)
+    arrs[4] = arrs[4].astype(np.int32)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    yy, d, act, rt, ct = np.empty(m), np.empty(n), np.empty(m), np.empty(m), np.empty(n)
+    out = _Certificate()
+    lib().oracle_certify(m, n, *[_p(a) for a in arrs], float(prob.obj_offset),
+                         int(int(prob.sense) == -1), _p(x), _p(y), float(dual_tol), _p(yy),
+                         _p(d), _p(act), _p(rt), _p(ct), C.byref(out))
+    return Certificate(out.bound, out.perturbation, out.objective, out.col_violation,
+                       out.row_violation, bool(out.vacuous), d, act)
 
 
 @dataclass

@@ -19,6 +19,7 @@
 
 #include "../../include/sovlp.h"
 #include "pdlp_kernels.hpp"
+#include "certify_kernels.hpp"
 
 using namespace sov;
 
@@ -318,6 +319,121 @@ int sovlp_spmv_csr_f64(int rows, const int64_t* rp, const int32_t* ri, const dou
 int sovlp_spmv_csr_f32(int rows, const int64_t* rp, const int32_t* ri, const float* rx,
                        const float* x, float* y, int lanes_per_row, sovlp_stream_t stream) {
   return launch_spmv_vector<float>(rows, rp, ri, rx, x, y, lanes_per_row, S(stream));
+}
+
+// ---------------------------------------------------------------- certificate
+int sovlp_residual_workspace_bytes(int rows, int64_t* bytes) {
+  if (!bytes) return fail(1, "null argument");
+  if (rows < 0) return fail(1, "negative row count");
+  *bytes = 4 * ((int64_t)rows + 4);
+  return 0;
+}
+
+int sovlp_residual_f64(int rows, const int64_t* rp, const int32_t* ri, const double* rx,
+                       const double* x, const double* b, double* out, void* workspace,
+                       int64_t workspace_bytes, sovlp_stream_t stream) {
+  if (rows < 0) return fail(1, "negative row count");
+  if (rows == 0) return 0;
+  if (!rp || !ri || !rx || !x || !b || !out || !workspace) return fail(1, "null argument");
+  if (workspace_bytes < 4 * ((int64_t)rows + 4))
+    return fail(1, "workspace too small (sovlp_residual_workspace_bytes)");
+  if (reinterpret_cast<uintptr_t>(workspace) % 4) return fail(1, "workspace not 4-byte aligned");
+  int* cnt = static_cast<int*>(workspace);   // the count of long rows, then their list
+  hipStream_t st = S(stream);
+  const int groups = std::min(kCertMaxGroups * 4, (rows + kCertBlock - 1) / kCertBlock);
+  zero_ints_kernel<<<1, 64, 0, st>>>(cnt, 1);
+  HIPCHK(hipGetLastError());
+  long_list_kernel<<<groups, kCertBlock, 0, st>>>(rows, rp, cnt, cnt + 4);
+  HIPCHK(hipGetLastError());
+  residual_kernel<<<groups, kCertBlock, 0, st>>>(rows, rp, ri, rx, x, b, out);
+  HIPCHK(hipGetLastError());
+  long_lines_kernel<true><<<kCertMaxGroups, kCertBlock, 0, st>>>(cnt, cnt + 4, rp, ri, rx, x, b,
+                                                                   1.0, out);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int cert_groups(int len) {
+  return std::max(1, std::min(kCertMaxGroups, (len + kCertBlock - 1) / kCertBlock));
+}
+
+int sovlp_certify_workspace_bytes(int m, int n, int64_t* bytes) {
+  if (!bytes) return fail(1, "null argument");
+  if (m < 0 || n < 0) return fail(1, "negative size");
+  *bytes = 8 * (2 * (int64_t)m + n + 15 * (int64_t)kCertMaxGroups + 8) +
+           4 * ((int64_t)m + n + 4);
+  return 0;
+}
+
+int sovlp_certify_f64(int m, int n, const int64_t* rp, const int32_t* ri, const double* rx,
+                      const int64_t* cp, const int32_t* ci, const double* cx, const double* c,
+                      const double* rl, const double* ru, const double* lo, const double* hi,
+                      double offset, int maximise, const double* x, const double* y,
+                      double dual_tol, void* workspace, int64_t workspace_bytes, double* d_out,
+                      double* act_out, sovlp_certificate* out, sovlp_stream_t stream) {
+  if (!out || !workspace) return fail(1, "null argument");
+  if (m < 0 || n < 0) return fail(1, "negative size");
+  if ((m > 0 && (!rp || !ri || !rx || !rl || !ru || !y)) ||
+      (n > 0 && (!cp || !ci || !cx || !c || !lo || !hi || !x)))
+    return fail(1, "null array");
+  int64_t need = 0;
+  sovlp_certify_workspace_bytes(m, n, &need);
+  if (workspace_bytes < need) return fail(1, "workspace too small (sovlp_certify_workspace_bytes)");
+  if (reinterpret_cast<uintptr_t>(workspace) % 8) return fail(1, "workspace not 8-byte aligned");
+  double* ws = static_cast<double*>(workspace);
+  CertPtrs P{};
+  P.m = m; P.n = n; P.maximise = maximise ? 1 : 0; P.dual_tol = dual_tol;
+  P.rp = rp; P.cp = cp; P.ri = ri; P.ci = ci; P.rx = rx; P.cx = cx;
+  P.c = c; P.rl = rl; P.ru = ru; P.lo = lo; P.hi = hi; P.x = x; P.y = y;
+  P.yy = ws;
+  P.d = d_out ? d_out : ws + m;
+  P.act = act_out ? act_out : ws + m + n;
+  P.p1 = ws + 2 * (int64_t)m + n;
+  P.p2 = P.p1 + 5 * kCertMaxGroups;
+  P.p3 = P.p2 + 6 * kCertMaxGroups;
+  P.res = P.p3 + 4 * kCertMaxGroups;
+  int* cnt = reinterpret_cast<int*>(P.res + 8);     // two counters, then the lists
+  int* list_r = cnt + 4;
+  int* list_c = list_r + m;
+  const int g1 = cert_groups(m), g2 = cert_groups(n);
+  hipStream_t st = S(stream);
+  // long rows' activities, then the rows (yy), long columns' d, the columns
+  zero_ints_kernel<<<1, 64, 0, st>>>(cnt, 2);
+  HIPCHK(hipGetLastError());
+  if (m > 0) {
+    long_list_kernel<<<g1, kCertBlock, 0, st>>>(m, rp, cnt, list_r);
+    HIPCHK(hipGetLastError());
+    long_lines_kernel<true><<<kCertMaxGroups, kCertBlock, 0, st>>>(cnt, list_r, rp, ri, rx, x,
+                                                                     nullptr, -1.0, P.act);
+    HIPCHK(hipGetLastError());
+  }
+  cert_rows_kernel<<<g1, kCertBlock, 0, st>>>(P);
+  HIPCHK(hipGetLastError());
+  if (n > 0) {
+    long_list_kernel<<<std::min(4 * kCertMaxGroups, (n + kCertBlock - 1) / kCertBlock),
+                       kCertBlock, 0, st>>>(n, cp, cnt + 1, list_c);
+    HIPCHK(hipGetLastError());
+    long_lines_kernel<false><<<kCertMaxGroups, kCertBlock, 0, st>>>(
+        cnt + 1, list_c, cp, ci, cx, P.yy, c, maximise ? -1.0 : 1.0, P.d);
+    HIPCHK(hipGetLastError());
+  }
+  cert_cols_kernel<<<g2, kCertBlock, 0, st>>>(P);
+  HIPCHK(hipGetLastError());
+  cert_colterms_kernel<<<g2, kCertBlock, 0, st>>>(P, g2);
+  HIPCHK(hipGetLastError());
+  cert_final_kernel<<<1, kCertBlock, 0, st>>>(P, g1, g2, offset);
+  HIPCHK(hipGetLastError());
+  double r[6];
+  HIPCHK(hipMemcpyAsync(r, P.res, sizeof(r), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  out->bound = r[0];
+  out->perturbation = r[1];
+  out->objective = r[2];
+  out->col_violation = r[3];
+  out->row_violation = r[4];
+  out->vacuous = r[5] > 0.0 ? 1 : 0;
+  out->reserved = 0;
+  return 0;
 }
 
 #define ELEMWISE_LAUNCH(kernel, len, ...)                                     \

@@ -51,6 +51,12 @@ class sovlp_info(C.Structure):
                 ("refine_round", I32), ("refine_scale", F64), ("start_tick", I64)]
 
 
+class sovlp_certificate(C.Structure):
+    _fields_ = [("bound", F64), ("perturbation", F64), ("objective", F64),
+                ("col_violation", F64), ("row_violation", F64), ("vacuous", I32),
+                ("reserved", I32)]
+
+
 # name -> (restype, argtypes); every entry point of include/sovlp.h
 _SIGS = {
     "sovlp_last_error": (C.c_char_p, []),
@@ -120,6 +126,12 @@ _SIGS = {
     "sovlp_server_release": (C.c_int, [P, C.c_int]),
     "sovlp_server_clock_hz": (C.c_int, [P, C.POINTER(F64)]),
     "sovlp_server_destroy": (C.c_int, [P]),
+    "sovlp_residual_workspace_bytes": (C.c_int, [C.c_int, C.POINTER(I64)]),
+    "sovlp_residual_f64": (C.c_int, [C.c_int, P, P, P, P, P, P, P, I64, P]),
+    "sovlp_certify_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.POINTER(I64)]),
+    "sovlp_certify_f64": (C.c_int, [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P, F64,
+                                    C.c_int, P, P, F64, P, I64, P, P,
+                                    C.POINTER(sovlp_certificate), P]),
 }
 
 

@@ -115,6 +115,11 @@ class Problem:
     def is_qp(self) -> bool:
         return self.Q is not None
 
+    @property
+    def integer_mask(self) -> np.ndarray:
+        """Columns that must take integer values (core/problem.py:151-153)."""
+        return (self.kind == VarKind.INTEGER) | (self.kind == VarKind.BINARY)
+
     def row_types(self):
         lo, hi = self.row_lb, self.row_ub
         flo, fhi = lo > -INF, hi < INF
