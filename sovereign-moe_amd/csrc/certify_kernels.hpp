@@ -117,15 +117,60 @@ __global__ void __launch_bounds__(kCertBlock)
 __global__ void __launch_bounds__(kCertBlock)
     long_list_kernel(int lines, const int64_t* __restrict__ ptr, int* count, int* list) {
   const long long stride = (long long)gridDim.x * kCertBlock;
-  for (long long i = (long long)blockIdx.x * kCertBlock + threadIdx.x; i < lines; i += stride)
-    if (long_line(ptr, i)) list[atomicAdd(count, 1)] = (int)i;
+  const int lane = (int)threadIdx.x % kWave;
+  // one atomic per wave (long lines cluster, and atomics on one address serialise)
+  for (long long i0 = (long long)blockIdx.x * kCertBlock; i0 < lines; i0 += stride) {
+    const long long i = i0 + threadIdx.x;
+    const bool lng = i < lines && long_line(ptr, i);
+    const unsigned long long mask = __ballot(lng);
+    int base = 0;
+    if (lane == 0 && mask != 0ull) base = atomicAdd(count, (int)__popcll(mask));
+    base = __shfl(base, 0);
+    if (lng) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
+  }
 }
 
 __global__ void zero_ints_kernel(int* p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+// Stage nonzeros [c0, c0 + len) of a line: products (and their exact errors)
+// into LDS, by the threads `first`.. of the workgroup
+template <bool COMP>
+__device__ __forceinline__ void stage_line(const int32_t* __restrict__ idx,
+                                           const double* __restrict__ val,
+                                           const double* __restrict__ x, long long c0, int len,
+                                           int first, double* sp, double* se) {
+#pragma clang fp contract(off)
+  constexpr int N = kCertBlock - kWave;
+  constexpr int PER = (kLineStage + N - 1) / N;
+  const int t = (int)threadIdx.x - first;
+  int jj[PER];
+  double a[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int k = t + u * N;
+    const int kc = k < len ? k : len - 1;         // in the line; discarded past len
+    jj[u] = idx[c0 + kc];
+    a[u] = val[c0 + kc];
+  }
+  double xv[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) xv[u] = x[jj[u]];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int k = t + u * N;
+    if (k < len) {
+      const double prod = a[u] * xv[u];
+      sp[k] = prod;
+      if constexpr (COMP) se[k] = dekker_err(a[u], xv[u], prod);
+    }
+  }
+}
+
 // The listed long lines, workgroup g taking entries g, g + gridDim.x, ...
+// Waves 1..3 stage chunk c + 1 while thread 0 sums chunk c in line order
+// (two LDS buffers, one barrier per chunk).
 //   COMP true : out[i] = sign * (b_i - A_i x), compensated (refine.py:55-84);
 //               b null means b = 0
 //   COMP false: out[i] = sign b_i - sum_p A_ip x_p, the plain sequential sum
@@ -136,53 +181,41 @@ __global__ void __launch_bounds__(kCertBlock)
                       const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                       const double* __restrict__ val, const double* __restrict__ x,
                       const double* __restrict__ b, double sign, double* __restrict__ out) {
-  __shared__ double sp[kLineStage];
-  __shared__ double se[COMP ? kLineStage : 1];
+  __shared__ double sp[2][kLineStage];
+  __shared__ double se[COMP ? 2 : 1][COMP ? kLineStage : 1];
   const int nl = *count;
+  const bool stager = threadIdx.x >= kWave;
   for (int q = blockIdx.x; q < nl; q += gridDim.x) {
     const long long line = list[q];
     const long long p0 = ptr[line], p1 = ptr[line + 1];
+    const int chunks = (int)((p1 - p0 + kLineStage - 1) / kLineStage);
+    auto chunk_len = [&](int c) {
+      const long long r = p1 - (p0 + (long long)c * kLineStage);
+      return (int)(r < kLineStage ? r : kLineStage);
+    };
+    if (stager) stage_line<COMP>(idx, val, x, p0, chunk_len(0), kWave, sp[0], se[0]);
+    __syncthreads();
     double s = 0.0, comp = 0.0;        // thread 0's accumulators
-    for (long long c0 = p0; c0 < p1; c0 += kLineStage) {
-      const int len = (int)(p1 - c0 < kLineStage ? p1 - c0 : kLineStage);
-      __syncthreads();                 // the last chunk's sum has read the stage
-      {
+    for (int c = 0; c < chunks; ++c) {
+      const int cur = c & 1;
+      if (stager) {
+        if (c + 1 < chunks)
+          stage_line<COMP>(idx, val, x, p0 + (long long)(c + 1) * kLineStage,
+                           chunk_len(c + 1), kWave, sp[cur ^ 1], se[COMP ? cur ^ 1 : 0]);
+      } else if (threadIdx.x == 0) {
 #pragma clang fp contract(off)
-        constexpr int PER = kLineStage / kCertBlock;
-        int jj[PER];
-        double a[PER];
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-          const int k = (int)threadIdx.x + u * kCertBlock;
-          const int kc = k < len ? k : len - 1;       // in the line; discarded past len
-          jj[u] = idx[c0 + kc];
-          a[u] = val[c0 + kc];
-        }
-        double xv[PER];
-#pragma unroll
-        for (int u = 0; u < PER; ++u) xv[u] = x[jj[u]];
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-          const int k = (int)threadIdx.x + u * kCertBlock;
-          if (k < len) {
-            const double prod = a[u] * xv[u];
-            sp[k] = prod;
-            if constexpr (COMP) se[k] = dekker_err(a[u], xv[u], prod);
-          }
-        }
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-#pragma clang fp contract(off)
+        const int len = chunk_len(c);
+        const double* bp = sp[cur];
+        const double* be = se[COMP ? cur : 0];
         int k = 0;
         for (; k + 8 <= len; k += 8) {
           double v[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) v[u] = sp[k + u];
+          for (int u = 0; u < 8; ++u) v[u] = bp[k + u];
           if constexpr (COMP) {
             double ev[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) ev[u] = se[k + u];
+            for (int u = 0; u < 8; ++u) ev[u] = be[k + u];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
               const double t = s + v[u];
@@ -198,17 +231,18 @@ __global__ void __launch_bounds__(kCertBlock)
         }
         for (; k < len; ++k) {
           if constexpr (COMP) {
-            const double prod = sp[k];
+            const double prod = bp[k];
             const double t = s + prod;
             const double bb = t - s;
             const double e = (s - (t - bb)) + (prod - bb);
             s = t;
-            comp += e + se[k];
+            comp += e + be[k];
           } else {
-            s += sp[k];
+            s += bp[k];
           }
         }
       }
+      __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma clang fp contract(off)
