@@ -1,0 +1,38 @@
+"""The solve entry point callers use (reference ``src/sovopt/cli.py:82-86, 163-166``).
+
+``solve(prob, method="pdlp", device="gpu", time_limit, gap, tol, verbose)``
+maps ``tol`` to ``eps_abs = eps_rel = tol`` and runs ``solve_pdlp``; ``"auto"``
+means PDLP here, the one method this package provides.  ``solve_file`` reads a
+model file first (``sovlp.io.read_model``), so
+``solve_file("model.mps", method="pdlp", device="gpu")`` is the library form
+of ``sovopt solve model.mps --method pdlp --device gpu``.
+"""
+
+from __future__ import annotations
+
+from .core.problem import Problem, Solution
+
+__all__ = ["solve", "solve_file"]
+
+
+def solve(prob: Problem, method: str = "auto", device: str = "auto",
+          time_limit: float = 300.0, gap: float = 1e-4, tol: float = 1e-8,
+          verbose: bool = False, **options) -> Solution:
+    """Solve an LP with PDLP on the GPU.  ``gap`` is accepted for the
+    reference's signature (it applies to branch-and-bound, not to an LP); the
+    reference's other methods and options (``simplex``, ``ipm``,
+    ``crossover``, ``sensitivity``, ``presolve``) are not provided and raise."""
+    from .lp.pdlp import PDLPParams, solve_pdlp
+    if method not in ("auto", "pdlp"):
+        raise NotImplementedError(f"method {method!r} is not provided (only 'pdlp')")
+    unsupported = [k for k, v in options.items() if v]
+    if unsupported:
+        raise NotImplementedError(f"options not provided: {', '.join(sorted(unsupported))}")
+    return solve_pdlp(prob, PDLPParams(device=device, eps_abs=tol, eps_rel=tol,
+                                       time_limit=time_limit, verbose=verbose))
+
+
+def solve_file(path, **kwargs) -> Solution:
+    """``solve(read_model(path), **kwargs)``."""
+    from .io import read_model
+    return solve(read_model(path), **kwargs)
