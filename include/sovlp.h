@@ -90,6 +90,38 @@ typedef struct {
 } sovlp_certificate;
 
 int sovlp_certify_workspace_bytes(int m, int n, int64_t *bytes);
+
+/* Batched node relaxation (src/sovopt/mip/bnr.py:176-301 BNREngine.solve_batch
+ * and mip/safebound.py:194-235 safe_dual_bound_batch): K LPs sharing A, c and
+ * the row bounds, each with its own column bounds L, U, advanced together by
+ * `iters` PDHG iterations with a per-node adaptive step (growth 1 - t^-g,
+ * damping 1 + t^-d), then bounded: bounds[k] = sum_i y_ik (rl_i | ru_i) +
+ * sum_j d_jk (L_jk | U_jk), d = c - A^T Y, -inf when a term points at an
+ * infinite bound.  Every (length, K) block is row-major, node index fastest.
+ * X, Y hold the start (X is clamped into each box) and receive the iterates;
+ * eta, omega (K) the step sizes (omega is not changed); empty[k] = 1 when
+ * some L_jk > U_jk + 1e-12.  1 <= K <= 256.  Everything device memory.
+ * dual_sign: 1 bounds with Y as the iteration leaves it, as the reference
+ * does (its PDHG dual is the negative of the textbook dual, so the bound is
+ * valid but weak, vacuous whenever a one-sided row's multiplier is nonzero);
+ * -1 bounds with -Y, the textbook dual, which tightens to the optimum as
+ * the iterates converge. */
+typedef struct {
+  int32_t m, n, K, iters;
+  const int64_t *rp, *cp;
+  const int32_t *ri, *ci;
+  const double *rx, *cx, *c, *rl, *ru;
+  const double *L, *U;
+  double *X, *Y, *eta, *omega;
+  double step_growth, step_damp;
+  double *bounds;
+  int32_t *empty;
+  double dual_sign;
+} sovlp_bnr_batch;
+
+int sovlp_bnr_workspace_bytes(int m, int n, int K, int64_t *bytes);
+int sovlp_bnr_solve_batch(const sovlp_bnr_batch *b, void *workspace, int64_t workspace_bytes,
+                          sovlp_stream_t stream);
 /* A: CSR (rp m+1, ri, rx) and CSC (cp n+1, ci, cx), int64 pointers; every
  * array a device pointer.  `d_out` (n, optional) receives c - A^T y in the
  * minimisation sense (before absorption), `act_out` (m, optional) the

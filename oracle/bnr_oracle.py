@@ -1,0 +1,128 @@
+"""CPU restatement of the batched node relaxation (test infrastructure only).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/`` as the checker of
+``sovlp_bnr_solve_batch``, never by the ``sovlp`` package.
+
+Restates ``BNREngine.solve_batch`` (reference ``src/sovopt/mip/bnr.py:176-301``)
+and ``safe_dual_bound_batch`` (``src/sovopt/mip/safebound.py:194-235``) in
+NumPy, with the same per-element operations.  The sparse products walk each
+row's nonzeros in CSR order for all nodes at once, as the reference's
+``_csr_spmm`` (``core/sparse.py:72-83``).  The per-node sums over rows (the
+step-size test's movement and interaction, the bound's two sums) are taken in
+the device's fixed order -- blocks of ``256 / KP`` rows (KP: K rounded up to a
+power of two), then the blocks in order -- where the reference's NumPy sums
+run down the rows.  The interaction sum can cancel heavily, so any other
+order moves the step sizes visibly; with the same order the whole batch is
+the device's bit for bit.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+INF = 1.0e30
+QUOTIENT_MAX = 2.0 ** 1000
+
+
+def spmm(rp, ri, rx, V, rows):
+    """``out[i, :] = sum_p rx[p] V[ri[p], :]`` over row i's nonzeros, in order."""
+    K = V.shape[1]
+    out = np.zeros((rows, K))
+    for i in range(rows):
+        acc = np.zeros(K)
+        for p in range(rp[i], rp[i + 1]):
+            acc = acc + rx[p] * V[ri[p]]
+        out[i] = acc
+    return out
+
+
+def block_rows(K: int) -> int:
+    """Rows per device workgroup for a batch of K nodes."""
+    kp = 1
+    while kp < K:
+        kp <<= 1
+    return 256 // kp
+
+
+def blocked_sum(V, rb: int):
+    """Per-column sum of V (rows, K): each block of rb rows in order from 0,
+    then the blocks in order from 0."""
+    total = np.zeros(V.shape[1])
+    for g in range(0, max(V.shape[0], 1), rb):
+        part = np.zeros(V.shape[1])
+        for r in range(g, min(g + rb, V.shape[0])):
+            part = part + V[r]
+        total = total + part
+    return total
+
+
+def safe_dual_bound_batch(prob, L, U, Y):
+    """(K,) bounds from the (m, K) duals; -inf where a term is vacuous."""
+    A = prob.A
+    aty = spmm(A.cp, A.ci, A.cx, Y, A.n)
+    d = prob.c[:, None] - aty
+    rl, ru = prob.row_lb[:, None], prob.row_ub[:, None]
+    ypos = Y > 0.0
+    row_bad = np.where(ypos, rl <= -INF, ru >= INF) & (Y != 0.0)
+    row_pick = np.where(ypos, rl, ru)
+    row_c = np.where(row_bad, 0.0, Y * np.where(row_bad, 0.0, row_pick))
+    dpos = d > 0.0
+    col_bad = np.where(dpos, L <= -INF, U >= INF) & (d != 0.0)
+    col_pick = np.where(dpos, L, U)
+    col_c = np.where(col_bad, 0.0, d * np.where(col_bad, 0.0, col_pick))
+    vac = row_bad.any(axis=0) | col_bad.any(axis=0)
+    rb = block_rows(Y.shape[1])
+    s = blocked_sum(row_c, rb) if row_c.shape[0] else np.zeros(Y.shape[1])
+    t = blocked_sum(col_c, rb)
+    return np.where(vac, -np.inf, s + t)
+
+
+def solve_batch(prob, L, U, iters=60, warm=None, eta=None, omega=None, growth=0.3, damp=0.6,
+                dual_sign=1.0):
+    """One batch visit: returns (bounds scaled, X, Y, eta, empty).  The bound
+    reads ``dual_sign * Y`` (1: as the reference; -1: textbook signs)."""
+    A = prob.A
+    m, n, K = A.m, A.n, L.shape[1]
+    L = np.asarray(L, dtype=np.float64)
+    U = np.asarray(U, dtype=np.float64)
+    empty = (L > U + 1e-12).any(axis=0)
+    X = np.zeros((n, K)) if warm is None else np.array(warm[0], dtype=np.float64)
+    Y = np.zeros((m, K)) if warm is None else np.array(warm[1], dtype=np.float64)
+    X = np.minimum(np.maximum(X, L), U)
+    eta = np.ones(K) if eta is None else np.array(eta, dtype=np.float64)
+    omega = np.ones(K) if omega is None else np.array(omega, dtype=np.float64)
+    AX = spmm(A.rp, A.ri, A.rx, X, m)
+    AtY = spmm(A.cp, A.ci, A.cx, Y, n)
+    c, rl, ru = prob.c[:, None], prob.row_lb[:, None], prob.row_ub[:, None]
+    attempts = 0
+    for _ in range(iters):
+        attempts += 1
+        tau = (eta / omega)[None, :]
+        sigma = (eta * omega)[None, :]
+        Xn = np.minimum(np.maximum(X - tau * (c + AtY), L), U)
+        dX = Xn - X
+        AdX = spmm(A.rp, A.ri, A.rx, dX, m)
+        W = Y + sigma * (AX + 2.0 * AdX)
+        Yn = W - sigma * np.minimum(np.maximum(W / sigma, rl), ru)
+        dY = Yn - Y
+        AtdY = spmm(A.cp, A.ci, A.cx, dY, n)
+        rb = block_rows(K)
+        inter = blocked_sum(dY * AdX, rb) if m else np.zeros(K)
+        dyy = blocked_sum(dY * dY, rb) if m else np.zeros(K)
+        dxx = blocked_sum(dX * dX, rb)
+        interaction = np.abs(inter)
+        movement = 0.5 * (omega * dxx + dyy / omega)
+        floor = np.maximum(movement, 1.0) / QUOTIENT_MAX
+        with np.errstate(divide="ignore", invalid="ignore"):
+            limit = np.where(interaction > 0.0, movement / np.maximum(interaction, floor), np.inf)
+        accept = (eta <= limit) | (movement == 0.0)
+        a = accept[None, :]
+        X = np.where(a, Xn, X)
+        Y = np.where(a, Yn, Y)
+        AX = np.where(a, AX + AdX, AX)
+        AtY = np.where(a, AtY + AtdY, AtY)
+        t = attempts + 1
+        eta = np.minimum((1.0 - t ** -growth) * limit, (1.0 + t ** -damp) * eta)
+        eta = np.maximum(np.where(np.isfinite(eta), eta, 1.0), 1e-14)
+    bounds = safe_dual_bound_batch(prob, L, U, dual_sign * Y)
+    return bounds, X, Y, eta, empty

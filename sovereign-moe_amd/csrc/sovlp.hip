@@ -20,6 +20,7 @@
 #include "../../include/sovlp.h"
 #include "pdlp_kernels.hpp"
 #include "certify_kernels.hpp"
+#include "bnr_kernels.hpp"
 
 using namespace sov;
 
@@ -433,6 +434,101 @@ int sovlp_certify_f64(int m, int n, const int64_t* rp, const int32_t* ri, const 
   out->row_violation = r[4];
   out->vacuous = r[5] > 0.0 ? 1 : 0;
   out->reserved = 0;
+  return 0;
+}
+
+// ------------------------------------------------- batched node relaxation
+static void bnr_shape(int K, int* KP, int* RB) {
+  int kp = 1;
+  while (kp < K) kp <<= 1;
+  *KP = kp;
+  *RB = kBnrBlock / kp;
+}
+
+static int64_t bnr_groups(int len, int RB) { return std::max<int64_t>(1, (len + RB - 1) / RB); }
+
+int sovlp_bnr_workspace_bytes(int m, int n, int K, int64_t* bytes) {
+  if (!bytes) return fail(1, "null argument");
+  if (m < 0 || n < 1 || K < 1 || K > 256) return fail(1, "need m >= 0, n >= 1, 1 <= K <= 256");
+  int KP, RB;
+  bnr_shape(K, &KP, &RB);
+  const int64_t gm = bnr_groups(m, RB), gn = bnr_groups(n, RB);
+  const int64_t parts = 3 * (int64_t)K * (gn + gm + (gm + gn));
+  *bytes = 8 * (4 * (int64_t)n * K + 4 * (int64_t)m * K + parts) + 4 * (int64_t)K + 256;
+  return 0;
+}
+
+int sovlp_bnr_solve_batch(const sovlp_bnr_batch* b, void* workspace, int64_t workspace_bytes,
+                          sovlp_stream_t stream) {
+  if (!b || !workspace) return fail(1, "null argument");
+  const int m = b->m, n = b->n, K = b->K;
+  int64_t need = 0;
+  if (int rc = sovlp_bnr_workspace_bytes(m, n, K, &need)) return rc;
+  if (workspace_bytes < need) return fail(1, "workspace too small (sovlp_bnr_workspace_bytes)");
+  if (reinterpret_cast<uintptr_t>(workspace) % 8) return fail(1, "workspace not 8-byte aligned");
+  if (b->iters < 0) return fail(1, "negative iteration count");
+  // index and value arrays may be null when A has no nonzeros (never read then)
+  if (!b->cp || !b->c || !b->L || !b->U || !b->X || !b->eta || !b->omega || !b->bounds ||
+      !b->empty || (m > 0 && (!b->rp || !b->rl || !b->ru || !b->Y)))
+    return fail(1, "null array");
+  int KP, RB;
+  bnr_shape(K, &KP, &RB);
+  const int gm = (int)bnr_groups(m, RB), gn = (int)bnr_groups(n, RB);
+  double* w = static_cast<double*>(workspace);
+  BnrPtrs P{};
+  P.m = m; P.n = n; P.K = K; P.KP = KP; P.RB = RB;
+  P.rp = b->rp; P.cp = b->cp; P.ri = b->ri; P.ci = b->ci; P.rx = b->rx; P.cx = b->cx;
+  P.c = b->c; P.rl = b->rl; P.ru = b->ru; P.L = b->L; P.U = b->U;
+  P.X = b->X; P.Y = b->Y; P.eta = b->eta; P.omega = b->omega;
+  P.growth = b->step_growth; P.damp = b->step_damp;
+  if (b->dual_sign != 1.0 && b->dual_sign != -1.0) return fail(1, "dual_sign must be 1 or -1");
+  P.ysign = b->dual_sign;
+  const int64_t nk = (int64_t)n * K, mk = (int64_t)m * K;
+  P.Xn = w; P.dX = w + nk; P.AtY = w + 2 * nk; P.AtdY = w + 3 * nk;
+  P.Yn = w + 4 * nk; P.dY = P.Yn + mk; P.AX = P.Yn + 2 * mk; P.AdX = P.Yn + 3 * mk;
+  double* part_p = P.Yn + 4 * mk;                       // primal pass: gn groups
+  double* part_d = part_p + 3 * (int64_t)K * gn;        // dual pass: gm groups
+  double* part_b = part_d + 3 * (int64_t)K * gm;        // bound terms: gm + gn groups
+  P.accept = reinterpret_cast<int*>(part_b + 3 * (int64_t)K * (gm + gn));
+  P.empty = b->empty;
+  P.bounds = b->bounds;
+  hipStream_t st = S(stream);
+  const int gk = (K + 63) / 64;
+  bnr_zero_kernel<<<gk, 64, 0, st>>>(P.empty, K);
+  HIPCHK(hipGetLastError());
+  bnr_start_kernel<<<gn, kBnrBlock, 0, st>>>(P);
+  HIPCHK(hipGetLastError());
+  if (m > 0) bnr_spmm_kernel<<<gm, kBnrBlock, 0, st>>>(m, K, KP, RB, P.rp, P.ri, P.rx, P.X, P.AX);
+  bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, KP, RB, P.cp, P.ci, P.cx, P.Y, P.AtY);
+  HIPCHK(hipGetLastError());
+  if (m == 0) HIPCHK(hipMemsetAsync(P.AtdY, 0, sizeof(double) * nk, st));   // A^T dY = 0
+  BnrPtrs Pp = P, Pd = P, Pb = P;
+  Pp.part = part_p;
+  Pd.part = part_d;
+  Pb.part = part_b;
+  for (int it = 0; it < b->iters; ++it) {
+    bnr_primal_kernel<<<gn, kBnrBlock, 0, st>>>(Pp);
+    if (m > 0) {
+      bnr_spmm_kernel<<<gm, kBnrBlock, 0, st>>>(m, K, KP, RB, P.rp, P.ri, P.rx, P.dX, P.AdX);
+      bnr_dual_kernel<<<gm, kBnrBlock, 0, st>>>(Pd);
+      bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, KP, RB, P.cp, P.ci, P.cx, P.dY, P.AtdY);
+    }
+    const double t = (double)(it + 2);
+    bnr_decide_kernel<<<gk, 64, 0, st>>>(P, part_p, gn, part_d, m > 0 ? gm : 0,
+                                         1.0 - std::pow(t, -b->step_growth),
+                                         1.0 + std::pow(t, -b->step_damp));
+    bnr_update_kernel<<<gn + (m > 0 ? gm : 0), kBnrBlock, 0, st>>>(P, gn);
+    HIPCHK(hipGetLastError());
+  }
+  // the bound from the final Y: A^T Y into AtdY, then the terms
+  if (m > 0) {
+    bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, KP, RB, P.cp, P.ci, P.cx, P.Y, P.AtdY);
+  } else {
+    HIPCHK(hipMemsetAsync(P.AtdY, 0, sizeof(double) * nk, st));
+  }
+  bnr_bound_terms_kernel<<<(m > 0 ? gm : 0) + gn, kBnrBlock, 0, st>>>(Pb, m > 0 ? gm : 0);
+  bnr_bound_final_kernel<<<gk, 64, 0, st>>>(Pb, m > 0 ? gm : 0, gn);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

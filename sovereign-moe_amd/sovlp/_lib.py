@@ -57,6 +57,14 @@ class sovlp_certificate(C.Structure):
                 ("reserved", I32)]
 
 
+class sovlp_bnr_batch(C.Structure):
+    _fields_ = [("m", I32), ("n", I32), ("K", I32), ("iters", I32), ("rp", P), ("cp", P),
+                ("ri", P), ("ci", P), ("rx", P), ("cx", P), ("c", P), ("rl", P), ("ru", P),
+                ("L", P), ("U", P), ("X", P), ("Y", P), ("eta", P), ("omega", P),
+                ("step_growth", F64), ("step_damp", F64), ("bounds", P), ("empty", P),
+                ("dual_sign", F64)]
+
+
 # name -> (restype, argtypes); every entry point of include/sovlp.h
 _SIGS = {
     "sovlp_last_error": (C.c_char_p, []),
@@ -127,6 +135,8 @@ _SIGS = {
     "sovlp_server_clock_hz": (C.c_int, [P, C.POINTER(F64)]),
     "sovlp_server_destroy": (C.c_int, [P]),
     "sovlp_residual_workspace_bytes": (C.c_int, [C.c_int, C.POINTER(I64)]),
+    "sovlp_bnr_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(I64)]),
+    "sovlp_bnr_solve_batch": (C.c_int, [C.POINTER(sovlp_bnr_batch), P, I64, P]),
     "sovlp_residual_f64": (C.c_int, [C.c_int, P, P, P, P, P, P, P, I64, P]),
     "sovlp_certify_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.POINTER(I64)]),
     "sovlp_certify_f64": (C.c_int, [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P, F64,

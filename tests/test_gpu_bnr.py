@@ -1,0 +1,68 @@
+"""Batched node relaxation on the GPU against the oracle.
+
+The shared-A products are the oracle's bit for bit (same per-element order);
+the per-node sums (step-size test, bound) are fixed trees on the device and
+row-order sums in the oracle, so bounds and iterates agree to rounding
+(1e-9 relative), flags exactly.  Every bound is checked valid against HiGHS.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import bnr_oracle as bo
+from sovlp.core.problem import Problem
+from sovlp.core.sparse import SparseMatrix
+from sovlp.workloads import known_optimum_lp
+
+from test_bnr_oracle import highs_optima, node_boxes
+
+pytestmark = pytest.mark.gpu
+
+
+def _close(a, b, rel=1e-9):
+    a, b = np.asarray(a), np.asarray(b)
+    fin = np.isfinite(b)
+    assert np.array_equal(np.isfinite(a), fin) and np.array_equal(a[~fin], b[~fin])
+    assert np.all(np.abs(a[fin] - b[fin]) <= rel * np.maximum(1.0, np.abs(b[fin]))), (a, b)
+
+
+@pytest.mark.parametrize("m_eq,K,iters,textbook", [(30, 5, 60, False), (30, 5, 60, True),
+                                                   (15, 64, 60, True), (15, 130, 20, False),
+                                                   (15, 1, 200, True)])
+def test_batches_match_the_oracle(m_eq, K, iters, textbook):
+    print(f"[gpu] bnr m_eq={m_eq} K={K} iters={iters} textbook={textbook}", flush=True)
+    from sovlp.mip.bnr import BNRConfig, BNREngine
+    k = known_optimum_lp(30, 60, 3, seed=4, m_eq=m_eq)
+    p = k.problem
+    L, U = node_boxes(p, K, seed=3)
+    if K > 2:
+        L[5, 2], U[5, 2] = 3.0, 1.0                        # an empty node
+    sign = -1.0 if textbook else 1.0
+    eng = BNREngine(p, config=BNRConfig(batch=K, iters=iters, textbook_duals=textbook))
+    r = eng.solve_batch(L, U)
+    b, X, Y, eta, empty = bo.solve_batch(p, L, U, iters=iters, dual_sign=sign)
+    assert np.array_equal(r.infeasible, empty)
+    _close(r.bounds, np.where(empty, np.inf, b))
+    _close(r.x, X)
+    _close(eng.eta[:K], eta)
+    # a second visit, warm from the first, with the step sizes carried over
+    Xw, Yw = eng.iterates(K)
+    r2 = eng.solve_batch(L, U, warm=(Xw, Yw))
+    b2, X2, *_ = bo.solve_batch(p, L, U, iters=iters, warm=(X, Y), eta=eta, dual_sign=sign)
+    _close(r2.bounds, np.where(empty, np.inf, b2))
+    _close(r2.x, X2)
+    opt = highs_optima(p, L, U)
+    ok = ~empty
+    assert np.all(r2.bounds[ok] <= opt[ok] + 1e-9 * np.maximum(1.0, np.abs(opt[ok])))
+
+
+def test_a_model_without_rows():
+    from sovlp.mip.bnr import BNRConfig, BNREngine
+    one = Problem(A=SparseMatrix.from_triplets(np.array([], np.int64), np.array([], np.int64),
+                                               np.array([]), 0, 2),
+                  c=np.array([1.0, -2.0]), row_lb=np.array([]), row_ub=np.array([]),
+                  col_lb=np.array([1.0, 0.0]), col_ub=np.array([5.0, 3.0]))
+    r = BNREngine(one, config=BNRConfig(batch=2, iters=5)).solve_batch(
+        np.array([[1.0, 2.0], [0.0, 0.0]]), np.array([[5.0, 5.0], [3.0, 1.0]]))
+    # min x0 - 2 x1: x0 at its lower bound, x1 at its upper bound
+    np.testing.assert_array_equal(r.bounds, [1.0 - 6.0, 2.0 - 2.0])
