@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="wall budget of the CPU baseline sample (0: skip)")
     ap.add_argument("--spmv", action="store_true", help="also time the SpMV of the 200k x 1M LP")
+    ap.add_argument("--bnr", action="store_true",
+                    help="also time a batched node relaxation: 128 nodes of the 500 x 2000 LP, "
+                         "60 PDHG iterations, and the oracle on a sample")
     ap.add_argument("--certify", action="store_true",
                     help="also time the certificate (bound, reduced costs, compensated row "
                          "activities) of a point of the 200k x 1M LP, and the C oracle on it")
@@ -122,6 +125,54 @@ def robust_rate(problems, par, tuner, a) -> dict:
     return {"value": opt * a.robust_sample / wall, "unit": "LPs/s", "optimal": opt,
             "ms_per_step": wall / a.robust_sample * 1e3,
             "refinement": refinement_config(p2, a.precision)}
+
+
+def bnr_rate(K: int = 128, iters: int = 60, reps: int = 10) -> dict:
+    """Batched node relaxation: K nodes of the 500 x 2000 LP (random sub-boxes),
+    `iters` PDHG iterations per call; device time per call (wall, synchronised),
+    node-iterations per second, algorithmic bytes per iteration, and the oracle
+    on 5 iterations of the same batch (one host thread, NumPy)."""
+    import numpy as np
+    import torch
+    from sovlp.mip.bnr import BNRConfig, BNREngine
+    from sovlp.workloads import small_lp
+    p = small_lp(0).problem
+    rng = np.random.default_rng(5)
+    L = np.repeat(p.col_lb[:, None], K, 1).copy()
+    U = np.repeat(p.col_ub[:, None], K, 1).copy()
+    for q in range(1, K):
+        js = rng.choice(p.n, 20, replace=False)
+        U[js, q] = np.minimum(U[js, q], rng.uniform(0.0, 5.0, 20))
+    eng = BNREngine(p, config=BNRConfig(batch=K, iters=iters, textbook_duals=True))
+    eng.solve_batch(L, U)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        r = eng.solve_batch(L, U)
+    torch.cuda.synchronize()
+    s = (time.perf_counter() - t) / reps
+    m, n, nnz = p.m, p.n, p.A.nnz
+    # per iteration: two shared-A products (A once, K gathered values per
+    # nonzero, K outputs per row), the primal (6 values per (j, k)), dual (5 per
+    # (i, k)) and update (4 per entry) passes
+    prod = lambda rows: 12 * nnz + 8 * (rows + 1) + 8 * nnz * K + 8 * rows * K  # noqa: E731
+    per_iter = prod(m) + prod(n) + 8 * K * (6 * n + 5 * m + 4 * (n + m))
+    out = {"lp": f"{m} x {n}, {nnz} nonzeros", "nodes": K, "iters": iters,
+           "ms_per_call": s * 1e3, "node_iterations_per_s": K * iters / s,
+           "algorithmic_bytes_per_iteration": per_iter,
+           "achieved_gbs": per_iter * iters / s / 1e9,
+           "frac_of_hbm_peak": per_iter * iters / s / 1e9 / HBM_PEAK_GBS,
+           "bounds_finite": int(np.isfinite(r.bounds).sum())}
+    try:
+        from oracle import bnr_oracle
+        t = time.perf_counter()
+        bnr_oracle.solve_batch(p, L, U, iters=5, dual_sign=-1.0)
+        cpu = (time.perf_counter() - t) / 5
+        out["cpu_baseline"] = {"node_iterations_per_s": K / cpu, "cores": 1, "kind": "port",
+                               "sample": f"5 iterations of the same {K}-node batch"}
+    except ImportError as e:
+        out["cpu_baseline"] = {"error": str(e)}
+    return out
 
 
 def certify_rate(reps: int = 20) -> dict:
@@ -405,6 +456,9 @@ def main():
         from sovlp.spmv_bench import skewed_roofline
         res, gen = skewed_roofline()
         out["spmv"] = res
+    if a.bnr:
+        out["bnr"] = bnr_rate()
+        log(f"bnr: {out['bnr']}")
     if a.certify:
         out["certify"] = certify_rate()
         log(f"certificate: {out['certify']}")

@@ -45,14 +45,23 @@ def block_rows(K: int) -> int:
 
 
 def blocked_sum(V, rb: int):
-    """Per-column sum of V (rows, K): each block of rb rows in order from 0,
-    then the blocks in order from 0."""
-    total = np.zeros(V.shape[1])
-    for g in range(0, max(V.shape[0], 1), rb):
-        part = np.zeros(V.shape[1])
-        for r in range(g, min(g + rb, V.shape[0])):
+    """Per-column sum of V (rows, K) in the device's order: each block of rb
+    rows in order from 0; then lane t (of 256) adds blocks t, t + 256, ... in
+    order from 0, and the 256 lane sums are added in order from 0."""
+    K = V.shape[1]
+    nblocks = max(1, -(-V.shape[0] // rb))
+    parts = []
+    for g in range(nblocks):
+        part = np.zeros(K)
+        for r in range(g * rb, min((g + 1) * rb, V.shape[0])):
             part = part + V[r]
-        total = total + part
+        parts.append(part)
+    total = np.zeros(K)
+    for t in range(256):
+        lane = np.zeros(K)
+        for g in range(t, nblocks, 256):
+            lane = lane + parts[g]
+        total = total + lane
     return total
 
 

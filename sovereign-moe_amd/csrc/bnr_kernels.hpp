@@ -131,20 +131,37 @@ __global__ void __launch_bounds__(kBnrBlock) bnr_dual_kernel(BnrPtrs P) {
   bnr_group_sums(P, 2, v, lds);
 }
 
-// One thread per node: the step-size test and update (bnr.py:265-288).
+// Sum over groups g of part[(g * 3 + slot) * K + k] by one workgroup per
+// node: lane t adds groups t, t + 256, ... in order, then lane 0 adds the 256
+// lane sums in order (the oracle sums in this same order).
+__device__ __forceinline__ double bnr_groups_sum(const double* part, int groups, int slot, int K,
+                                                 int k, double* lds) {
+#pragma clang fp contract(off)
+  double a = 0.0;
+  for (int g = threadIdx.x; g < groups; g += kBnrBlock)
+    a += part[((long long)g * 3 + slot) * K + k];
+  __syncthreads();
+  lds[threadIdx.x] = a;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int t = 0; t < kBnrBlock; ++t) s += lds[t];
+  return s;                            // lane 0's value is the sum
+}
+
+// One workgroup per node: the step-size test and update (bnr.py:265-288).
 // gp: groups of the primal pass, gd: of the dual pass; gfac = 1 - t^-growth and
 // dfac = 1 + t^-damp (t = attempts + 1), computed on the host with its libm.
-__global__ void bnr_decide_kernel(BnrPtrs P, const double* part_p, int gp, const double* part_d,
-                                  int gd, double gfac, double dfac) {
+__global__ void __launch_bounds__(kBnrBlock)
+    bnr_decide_kernel(BnrPtrs P, const double* part_p, int gp, const double* part_d, int gd,
+                      double gfac, double dfac) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P.K) return;
-  double dxx = 0.0, dyy = 0.0, inter = 0.0;
-  for (int g = 0; g < gp; ++g) dxx += part_p[(long long)g * 3 * P.K + k];
-  for (int g = 0; g < gd; ++g) {
-    dyy += part_d[(long long)g * 3 * P.K + k];
-    inter += part_d[((long long)g * 3 + 1) * P.K + k];
-  }
+  __shared__ double lds[kBnrBlock];
+  const int k = blockIdx.x;
+  const double dxx = bnr_groups_sum(part_p, gp, 0, P.K, k, lds);
+  const double dyy = bnr_groups_sum(part_d, gd, 0, P.K, k, lds);
+  const double inter = bnr_groups_sum(part_d, gd, 1, P.K, k, lds);
+  if (threadIdx.x != 0) return;
   const double omega = P.omega[k], eta = P.eta[k];
   const double interaction = fabs(inter);
   const double movement = 0.5 * (omega * dxx + dyy / omega);
@@ -227,22 +244,26 @@ __global__ void __launch_bounds__(kBnrBlock) bnr_bound_terms_kernel(BnrPtrs P, i
   }
 }
 
-// per node: row sum plus column sum (each over its groups in order), -inf
-// when vacuous (safebound.py:231-232)
-__global__ void bnr_bound_final_kernel(BnrPtrs P, int groups_m, int groups_n) {
+// per node: row sum plus column sum (each over its groups, in the order of
+// bnr_groups_sum), -inf when vacuous (safebound.py:231-232)
+__global__ void __launch_bounds__(kBnrBlock)
+    bnr_bound_final_kernel(BnrPtrs P, int groups_m, int groups_n) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= P.K) return;
-  double rs = 0.0, cs = 0.0, bad = 0.0;
-  for (int g = 0; g < groups_m; ++g) {
-    rs += P.part[((long long)g * 3) * P.K + k];
+  __shared__ double lds[kBnrBlock];
+  const int k = blockIdx.x;
+  const double rs = bnr_groups_sum(P.part, groups_m, 0, P.K, k, lds);
+  const double cs = bnr_groups_sum(P.part + (long long)groups_m * 3 * P.K, groups_n, 0, P.K, k,
+                                   lds);
+  double bad = 0.0;
+  for (int g = threadIdx.x; g < groups_m + groups_n; g += kBnrBlock)
     bad = fmax(bad, P.part[((long long)g * 3 + 1) * P.K + k]);
+  __syncthreads();
+  lds[threadIdx.x] = bad;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int t = 0; t < kBnrBlock; ++t) bad = fmax(bad, lds[t]);
+    P.bounds[k] = bad > 0.0 ? -__builtin_inf() : rs + cs;
   }
-  for (int g = groups_m; g < groups_m + groups_n; ++g) {
-    cs += P.part[((long long)g * 3) * P.K + k];
-    bad = fmax(bad, P.part[((long long)g * 3 + 1) * P.K + k]);
-  }
-  P.bounds[k] = bad > 0.0 ? -__builtin_inf() : rs + cs;
 }
 
 __global__ void bnr_zero_kernel(int* p, int n) {
