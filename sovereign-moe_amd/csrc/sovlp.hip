@@ -984,17 +984,24 @@ template <class T>
 int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, const int32_t* blocks,
                    void* ws, int64_t ws_bytes, int64_t hist_cap, hipStream_t st, sovlp_batch* h) {
   if ((size_t)ws_bytes < batch_ws_T<T>(count, lps, hist_cap)) return fail(1, "workspace too small");
-  // descriptors grouped by workgroup size (one launch per group); pos[i] is
-  // LP i's place in the device array
+  // descriptors grouped by workgroup size (one launch per group), largest LP
+  // first within a group, so that the longest workgroups of a launch start
+  // first and do not extend its tail; pos[i] is LP i's place in the device
+  // array (every output stays indexed by i)
   const int kBlocks[3] = {64, 128, 256};
   std::vector<int> order, pos(count), gcount(3, 0);
-  for (int g = 0; g < 3; ++g)
+  for (int g = 0; g < 3; ++g) {
+    std::vector<int> idx;
     for (int i = 0; i < count; ++i)
-      if (blocks[i] == kBlocks[g]) {
-        pos[i] = (int)order.size();
-        order.push_back(i);
-        ++gcount[g];
-      }
+      if (blocks[i] == kBlocks[g]) idx.push_back(i);
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return lps[a].nnz > lps[b].nnz; });
+    for (int i : idx) {
+      pos[i] = (int)order.size();
+      order.push_back(i);
+      ++gcount[g];
+    }
+  }
   if ((int)order.size() != count) return fail(1, "every LP's block must be 64, 128 or 256");
   Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
   LpDev<T>* d_lps = cv.take<LpDev<T>>(count);
