@@ -827,7 +827,7 @@ void launch_run(int block, const LpDev<T>* lps, int count, const StepCtl* ctl, u
   else
     pdlp_run_kernel<T, 256, T256><<<count, 256, 0, st>>>(lps, ctl, done, phases);
 }
-constexpr int kPhasesPerLaunch = 128;  // run kernel: phases per LP per launch
+constexpr int kPhasesPerLaunch = 1024;  // run kernel: phases per LP per launch (128: 2.9 % slower)
 
 template <class T>
 void fill_lp(LpDev<T>& L, const sovlp_lp& lp) {
