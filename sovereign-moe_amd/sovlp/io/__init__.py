@@ -1,22 +1,21 @@
-"""Model files (reference ``src/sovopt/io/__init__.py:12-25``): MPS in and out.
-
-``read_model`` picks the reader by extension, looking through compression
-suffixes.  The CPLEX LP format (``.lp``) is not provided: it raises
-``NotImplementedError`` rather than being misread as MPS.
+"""Model files (reference ``src/sovopt/io/__init__.py:12-25``): MPS in and out,
+CPLEX LP in.  ``read_model`` picks the reader by extension, looking through
+compression suffixes: ``.lp`` to ``read_lp``, everything else to ``read_mps``.
 """
 
+from .lp_format import LPFormatError, read_lp
 from .mps import MPSError, read_mps, write_mps
 
-__all__ = ["read_model", "read_mps", "write_mps", "MPSError"]
+__all__ = ["read_model", "read_mps", "write_mps", "read_lp", "MPSError", "LPFormatError"]
 
 
 def read_model(path, name=None):
-    """Read a model file; ``.mps`` (optionally ``.gz``, ``.bz2``, ``.xz``) only."""
+    """Read a model file (``.mps`` or ``.lp``, optionally ``.gz``, ``.bz2``, ``.xz``)."""
     base = str(path).lower()
     for suf in (".gz", ".bz2", ".xz", ".lzma"):
         if base.endswith(suf):
             base = base[: -len(suf)]
             break
     if base.endswith(".lp"):
-        raise NotImplementedError("the CPLEX LP format is not provided; convert the model to MPS")
+        return read_lp(path, name=name)
     return read_mps(path, name=name)

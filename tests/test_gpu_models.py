@@ -16,10 +16,11 @@ pytestmark = pytest.mark.gpu
 MODELS = __file__.rsplit("/", 1)[0] + "/golden/models/"
 
 
-def test_testprob_mps_solves_to_its_known_optimum():
-    print("[gpu] models: testprob.mps", flush=True)
+@pytest.mark.parametrize("fname", ["testprob.mps", "testprob.lp"])
+def test_testprob_solves_to_its_known_optimum(fname):
+    print(f"[gpu] models: {fname}", flush=True)
     from sovlp.solve import solve_file
-    s = solve_file(MODELS + "testprob.mps", method="pdlp", device="gpu")
+    s = solve_file(MODELS + fname, method="pdlp", device="gpu")
     assert s.status == Status.OPTIMAL
     assert abs(s.objective - 16.0) < 1e-7
     assert np.allclose(s.x, [0.0, -1.0, 6.0], atol=1e-6)

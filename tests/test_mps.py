@@ -194,5 +194,5 @@ def test_malformed_files_name_their_line(tmp_path):
     f.write_text("NAME X\nROWS\n N  OBJ\n Q  R1\nENDATA\n")
     with pytest.raises(MPSError, match="unknown row type"):
         read_mps(str(f))
-    with pytest.raises(NotImplementedError):
-        read_model(str(tmp_path / "model.lp"))
+    with pytest.raises(FileNotFoundError):
+        read_model(str(tmp_path / "model.lp"))          # .lp goes to the LP reader
