@@ -10,7 +10,7 @@ row's nonzeros in CSR order for all nodes at once, as the reference's
 ``_csr_spmm`` (``core/sparse.py:72-83``).  The per-node sums over rows (the
 step-size test's movement and interaction, the bound's two sums) are taken in
 the device's fixed order -- blocks of ``256 / KP`` rows (KP: K rounded up to a
-power of two), then the blocks in order -- where the reference's NumPy sums
+power of two), then the blocks over 256 lanes in a fixed order -- where the reference's NumPy sums
 run down the rows.  The interaction sum can cancel heavily, so any other
 order moves the step sizes visibly; with the same order the whole batch is
 the device's bit for bit.

@@ -18,6 +18,7 @@
 namespace sov {
 
 constexpr int kBnrBlock = 256;
+constexpr int kBnrLanes = 256;    // lanes of the per-node cross-group sums (64: slower)
 constexpr double kQuotientMax = 0x1p1000;   // core/tolerances.py:23
 
 struct BnrPtrs {
@@ -54,7 +55,21 @@ __global__ void __launch_bounds__(kBnrBlock)
   if (row >= rows || k >= K) return;
   double acc = 0.0;
   const long long p1 = rp[row + 1];
-  for (long long p = rp[row]; p < p1; ++p) acc += rx[p] * V[(long long)ri[p] * K + k];
+  long long p = rp[row];
+  for (; p + 4 <= p1; p += 4) {        // 4 nonzeros' loads, then the sums in order
+    int jj[4];
+    double a[4], v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      jj[u] = ri[p + u];
+      a[u] = rx[p + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = V[(long long)jj[u] * K + k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += a[u] * v[u];
+  }
+  for (; p < p1; ++p) acc += rx[p] * V[(long long)ri[p] * K + k];
   out[row * K + k] = acc;
 }
 
@@ -138,25 +153,25 @@ __device__ __forceinline__ double bnr_groups_sum(const double* part, int groups,
                                                  int k, double* lds) {
 #pragma clang fp contract(off)
   double a = 0.0;
-  for (int g = threadIdx.x; g < groups; g += kBnrBlock)
+  for (int g = threadIdx.x; g < groups; g += kBnrLanes)
     a += part[((long long)g * 3 + slot) * K + k];
   __syncthreads();
   lds[threadIdx.x] = a;
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0)
-    for (int t = 0; t < kBnrBlock; ++t) s += lds[t];
+    for (int t = 0; t < kBnrLanes; ++t) s += lds[t];
   return s;                            // lane 0's value is the sum
 }
 
 // One workgroup per node: the step-size test and update (bnr.py:265-288).
 // gp: groups of the primal pass, gd: of the dual pass; gfac = 1 - t^-growth and
 // dfac = 1 + t^-damp (t = attempts + 1), computed on the host with its libm.
-__global__ void __launch_bounds__(kBnrBlock)
+__global__ void __launch_bounds__(kBnrLanes)
     bnr_decide_kernel(BnrPtrs P, const double* part_p, int gp, const double* part_d, int gd,
                       double gfac, double dfac) {
 #pragma clang fp contract(off)
-  __shared__ double lds[kBnrBlock];
+  __shared__ double lds[kBnrLanes];
   const int k = blockIdx.x;
   const double dxx = bnr_groups_sum(part_p, gp, 0, P.K, k, lds);
   const double dyy = bnr_groups_sum(part_d, gd, 0, P.K, k, lds);
@@ -246,22 +261,22 @@ __global__ void __launch_bounds__(kBnrBlock) bnr_bound_terms_kernel(BnrPtrs P, i
 
 // per node: row sum plus column sum (each over its groups, in the order of
 // bnr_groups_sum), -inf when vacuous (safebound.py:231-232)
-__global__ void __launch_bounds__(kBnrBlock)
+__global__ void __launch_bounds__(kBnrLanes)
     bnr_bound_final_kernel(BnrPtrs P, int groups_m, int groups_n) {
 #pragma clang fp contract(off)
-  __shared__ double lds[kBnrBlock];
+  __shared__ double lds[kBnrLanes];
   const int k = blockIdx.x;
   const double rs = bnr_groups_sum(P.part, groups_m, 0, P.K, k, lds);
   const double cs = bnr_groups_sum(P.part + (long long)groups_m * 3 * P.K, groups_n, 0, P.K, k,
                                    lds);
   double bad = 0.0;
-  for (int g = threadIdx.x; g < groups_m + groups_n; g += kBnrBlock)
+  for (int g = threadIdx.x; g < groups_m + groups_n; g += kBnrLanes)
     bad = fmax(bad, P.part[((long long)g * 3 + 1) * P.K + k]);
   __syncthreads();
   lds[threadIdx.x] = bad;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int t = 0; t < kBnrBlock; ++t) bad = fmax(bad, lds[t]);
+    for (int t = 0; t < kBnrLanes; ++t) bad = fmax(bad, lds[t]);
     P.bounds[k] = bad > 0.0 ? -__builtin_inf() : rs + cs;
   }
 }

@@ -514,7 +514,7 @@ int sovlp_bnr_solve_batch(const sovlp_bnr_batch* b, void* workspace, int64_t wor
       bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, KP, RB, P.cp, P.ci, P.cx, P.dY, P.AtdY);
     }
     const double t = (double)(it + 2);
-    bnr_decide_kernel<<<K, kBnrBlock, 0, st>>>(P, part_p, gn, part_d, m > 0 ? gm : 0,
+    bnr_decide_kernel<<<K, kBnrLanes, 0, st>>>(P, part_p, gn, part_d, m > 0 ? gm : 0,
                                          1.0 - std::pow(t, -b->step_growth),
                                          1.0 + std::pow(t, -b->step_damp));
     bnr_update_kernel<<<gn + (m > 0 ? gm : 0), kBnrBlock, 0, st>>>(P, gn);
@@ -527,7 +527,7 @@ int sovlp_bnr_solve_batch(const sovlp_bnr_batch* b, void* workspace, int64_t wor
     HIPCHK(hipMemsetAsync(P.AtdY, 0, sizeof(double) * nk, st));
   }
   bnr_bound_terms_kernel<<<(m > 0 ? gm : 0) + gn, kBnrBlock, 0, st>>>(Pb, m > 0 ? gm : 0);
-  bnr_bound_final_kernel<<<K, kBnrBlock, 0, st>>>(Pb, m > 0 ? gm : 0, gn);
+  bnr_bound_final_kernel<<<K, kBnrLanes, 0, st>>>(Pb, m > 0 ? gm : 0, gn);
   HIPCHK(hipGetLastError());
   return 0;
 }
