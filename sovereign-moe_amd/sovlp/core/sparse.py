@@ -19,7 +19,8 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["SparseMatrix", "coo_to_csc", "csc_to_csr", "IDX", "VAL"]
+__all__ = ["SparseMatrix", "coo_to_csc", "csc_to_csr", "locality_order", "relabel_columns",
+           "IDX", "VAL"]
 
 IDX = np.int32
 VAL = np.float64
@@ -154,3 +155,40 @@ class SparseMatrix:
         cols = np.repeat(np.arange(self.n), np.diff(self.cp))
         A[self.ci, cols] = self.cx
         return A
+
+
+def locality_order(A: SparseMatrix, hot_degree: int = 8) -> np.ndarray:
+    """A new label for each column, so that the gathers of ``A x`` and the rows of
+    ``A^T y`` hit cache: columns with at least ``hot_degree`` nonzeros first, by
+    degree descending, then the rest by the first row that holds them (ties by
+    old label). On a matrix whose few long columns are scattered over a long x,
+    the gathers of the long columns then come from a short prefix of x."""
+    deg = np.diff(A.cp)
+    first = np.full(A.n, np.iinfo(np.int64).max, dtype=np.int64)
+    has = deg > 0
+    first[has] = A.ci[A.cp[:-1][has]]
+    hot = deg >= hot_degree
+    order = np.lexsort((np.arange(A.n), first, np.where(hot, -deg, 0), ~hot))
+    new = np.empty(A.n, dtype=np.int64)
+    new[order] = np.arange(A.n)
+    return new
+
+
+def relabel_columns(A: SparseMatrix, new) -> SparseMatrix:
+    """A with column ``j`` renamed ``new[j]`` (a permutation of ``range(n)``).
+
+    Each row keeps its entries in their order and each column its entries, so
+    with ``x_new[new[j]] = x[j]`` every row sum of ``A x`` and every column sum of
+    ``A^T y``, taken in storage order, is bit for bit the same as before. (The
+    device's tile pass over ``A^T`` plans its tiles over the moved rows and may
+    reduce a row over a different number of lanes: those sums agree to
+    rounding.)"""
+    new = np.asarray(new, dtype=np.int64)
+    if new.shape != (A.n,) or not np.array_equal(np.sort(new), np.arange(A.n)):
+        raise ValueError("new must be a permutation of the column labels")
+    order = np.argsort(new)                     # old column at each new position
+    lens = np.diff(A.cp)[order]
+    cp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    take = np.arange(A.nnz, dtype=np.int64) + np.repeat(A.cp[:-1][order] - cp[:-1], lens)
+    return SparseMatrix(A.m, A.n, cp, A.ci[take], A.cx[take], A.rp.copy(),
+                        new[A.ri].astype(IDX), A.rx.copy())

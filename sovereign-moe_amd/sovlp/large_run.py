@@ -9,6 +9,7 @@ result is reported with its relative objective error. With more than 200k
 nonzeros the engine runs every phase over W workgroups (``team="grid"``).
 
     python -m sovlp.large_run [--m 200000 --n 1000000 --nnz 8000000] [--precision 64 32]
+                              [--column-order given locality]
 """
 
 from __future__ import annotations
@@ -18,7 +19,8 @@ import json
 import time
 
 
-def run(m: int, n: int, nnz: int, precision: int, tol: float, time_limit: float, seed: int = 1):
+def run(m: int, n: int, nnz: int, precision: int, tol: float, time_limit: float, seed: int = 1,
+        column_order: str = "given"):
     from .lp.pdlp import PDLPParams, solve_pdlp
     from .workloads import skewed_lp
     t = time.perf_counter()
@@ -26,10 +28,11 @@ def run(m: int, n: int, nnz: int, precision: int, tol: float, time_limit: float,
     gen = time.perf_counter() - t
     p = ko.problem
     par = PDLPParams(eps_abs=tol, eps_rel=tol, feas_cap=tol, device="gpu", precision=precision,
-                     time_limit=time_limit)
+                     time_limit=time_limit, column_order=column_order)
     s = solve_pdlp(p, par)
     f = ko.f_star
     out = dict(m=p.m, n=p.n, nnz=int(p.A.nnz), precision=precision, tol=tol,
+               column_order=column_order,
                status=s.status.name, iterations=int(s.iterations), seconds=s.time,
                device_seconds=s.info.device_seconds, workgroups=s.info.workgroups,
                restarts=int(s.info.restarts), objective=s.objective, f_star=f,
@@ -49,9 +52,12 @@ def main():
     ap.add_argument("--precision", type=int, nargs="+", default=[64, 32])
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--time-limit", type=float, default=240.0)
+    ap.add_argument("--column-order", nargs="+", default=["given"], choices=("given", "locality"))
     a = ap.parse_args()
-    for prec in a.precision:
-        print(json.dumps(run(a.m, a.n, a.nnz, prec, a.tol, a.time_limit)), flush=True)
+    for order in a.column_order:
+        for prec in a.precision:
+            print(json.dumps(run(a.m, a.n, a.nnz, prec, a.tol, a.time_limit, column_order=order)),
+                  flush=True)
 
 
 if __name__ == "__main__":

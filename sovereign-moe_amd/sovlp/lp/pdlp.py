@@ -27,6 +27,7 @@ import numpy as np
 from .. import _lib
 from ..core.backend import get_backend
 from ..core.problem import ObjSense, Problem, Solution, Status
+from ..core.sparse import locality_order, relabel_columns
 
 __all__ = ["PDLPParams", "PDLPInfo", "solve_pdlp", "DeviceLP", "upload_problem"]
 
@@ -65,6 +66,13 @@ class PDLPParams:
     # before that (0: 4096)
     refine_stall: int = 0
     refine_stall_open: int = 0
+    # "locality": solve with the columns relabelled by
+    # ``sovlp.core.sparse.locality_order`` (long columns first), which puts the
+    # gathers of A x on the long columns into a short prefix of x; x and the
+    # reduced costs come back in the caller's order. A x is bit for bit that of
+    # the given order; A^T y (whose tiles move with the columns) and the sums
+    # over columns agree to rounding.
+    column_order: str = "given"
 
     def resolved_refine(self, precision: int | None = None) -> tuple[int, float]:
         prec = int(self.precision if precision is None else precision)
@@ -268,8 +276,22 @@ def solve_pdlp(prob: Problem, params: PDLPParams | None = None) -> Solution:
     t0 = time.perf_counter()
     if params.scaling != "pdlp":
         raise ValueError("only scaling='pdlp' (Ruiz + Pock-Chambolle) is provided")
+    if params.column_order not in ("given", "locality"):
+        raise ValueError("column_order is 'given' or 'locality'")
     bk = get_backend(params.device)          # raises without a usable GPU
-    dlp = DeviceLP(prob, params.precision)
+    new = None
+    v0 = None
+    if params.column_order == "locality" and prob.n > 0:
+        import torch
+        new = locality_order(prob.A)
+        order = np.argsort(new)
+        prob = Problem(A=relabel_columns(prob.A, new), c=prob.c[order], row_lb=prob.row_lb,
+                       row_ub=prob.row_ub, col_lb=prob.col_lb[order], col_ub=prob.col_ub[order],
+                       kind=prob.kind[order], obj_offset=prob.obj_offset, sense=prob.sense,
+                       name=prob.name)
+        # the power iteration's start vector, relabelled with the columns
+        v0 = torch.from_numpy(np.random.default_rng(0).standard_normal(prob.n)[order]).cuda()
+    dlp = DeviceLP(prob, params.precision, v0=v0)
     eng = Engine(dlp, params)
     width = eng.width()
     try:
@@ -294,6 +316,8 @@ def solve_pdlp(prob: Problem, params: PDLPParams | None = None) -> Solution:
         x_h = x.cpu().numpy().copy()
         y_h = y.cpu().numpy().copy()
         d_h = d.cpu().numpy().copy()
+        if new is not None:
+            x_h, d_h = x_h[new], d_h[new]
         nh = min(int(inf.history_len), eng.hist_cap)
         hist_h = hist[: 5 * nh].cpu().numpy().reshape(nh, 5) if nh else np.zeros((0, 5))
         ref_rows = eng.refinement(int(inf.refine_rows))

@@ -2,7 +2,9 @@
 
 Times ``y = A x`` and ``y = Aᵀ x`` on the large skewed LP with HIP events on
 the stream the kernel runs on (and ``A x`` also as eight column panels, one per
-XCD: ``PanelSpmv``), and reports algorithmic bandwidth
+XCD: ``PanelSpmv``; and both with the columns relabelled by
+``locality_order``, which leaves every row sum of A x bit for bit the same),
+and reports algorithmic bandwidth
 
     B = Z·(s_val + s_idx) + (R+1)·s_ptr + C·s_x + R·s_y
 
@@ -18,6 +20,7 @@ import time
 import numpy as np
 
 from . import _lib
+from .core.sparse import locality_order, relabel_columns
 from .lp.pdlp import plan_tiles, tile_geometry
 
 HBM_PEAK_GBS = 8000.0
@@ -172,12 +175,17 @@ def skewed_roofline(reps: int = 50, seed: int = 1):
     t = time.perf_counter()
     A = skewed_lp(seed).problem.A
     gen_s = time.perf_counter() - t
+    B = relabel_columns(A, locality_order(A))
     out = []
     for prec in (64, 32):
         out.append(dict(op="A*x", kernel="tiled", **measure(A.rp, A.ri, A.rx, A.n, prec, reps)))
+        out.append(dict(op="A*x", kernel="tiled, columns in locality order",
+                        **measure(B.rp, B.ri, B.rx, B.n, prec, reps)))
         out.append(dict(op="A*x", kernel="8 column panels",
                         **measure_panels(A.rp, A.ri, A.rx, A.n, prec, 8, reps=reps)))
         out.append(dict(op="At*y", kernel="tiled", **measure(A.cp, A.ci, A.cx, A.m, prec, reps)))
+        out.append(dict(op="At*y", kernel="tiled, columns in locality order",
+                        **measure(B.cp, B.ci, B.cx, B.m, prec, reps)))
     return out, gen_s
 
 

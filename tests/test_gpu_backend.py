@@ -220,3 +220,34 @@ def test_column_panel_spmv_matches_oracle(precision, panels):
     ref = oracle.spmv(A.rp, A.ri, A.rx, x, m)
     tol = 1e-12 if precision == 64 else 1e-4
     np.testing.assert_allclose(y.cpu().numpy(), ref, rtol=tol, atol=tol)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_tiled_spmv_with_columns_in_locality_order_is_bitwise_the_same(dtype):
+    """Relabelling the columns (``locality_order``) changes where x is gathered
+    from, not one product or one addition of A x: y = A x is bit for bit that of
+    the given order. A^T y's rows move, so its tiles (and each row's reduction
+    tree) change: it agrees to rounding."""
+    from sovlp.core.sparse import locality_order, relabel_columns
+    from sovlp.workloads import skewed_lp_degrees
+    rng = np.random.default_rng(17)
+    m, n = 3000, 40_000
+    deg = skewed_lp_degrees(n, 300_000, dmax=2000)[rng.permutation(n)]
+    cols = np.repeat(np.arange(n), deg)
+    A = SparseMatrix.from_triplets(rng.integers(0, m, cols.size), cols,
+                                   rng.standard_normal(cols.size), m, n)
+    new = locality_order(A)
+    B = relabel_columns(A, new)
+    x = rng.standard_normal(n).astype(dtype)
+    xn = np.empty_like(x)
+    xn[new] = x
+    y = _tiled(A.rp, A.ri, A.rx, x, m, dtype)
+    yb = _tiled(B.rp, B.ri, B.rx, xn, m, dtype)
+    assert np.array_equal(y.view(np.uint8), yb.view(np.uint8))
+    v = rng.standard_normal(m).astype(dtype)
+    t = _tiled(A.cp, A.ci, A.cx, v, n, dtype)
+    tb = _tiled(B.cp, B.ci, B.cx, v, n, dtype)
+    f64 = dtype == np.float64
+    np.testing.assert_allclose(tb[new], t, rtol=1e-12 if f64 else 1e-4, atol=1e-12 if f64 else 2e-3)
+    np.testing.assert_allclose(y, A.matvec(x.astype(np.float64)),
+                               rtol=1e-10 if f64 else 1e-3, atol=1e-10 if f64 else 1e-3)

@@ -162,3 +162,31 @@ def test_skewed_lp_over_many_workgroups_matches_closed_form(precision):
     assert abs(s.objective - ko.f_star) <= tol * max(1.0, abs(ko.f_star))
     rv, bv, _ = ko.problem.violation(s.x)
     assert rv <= 1e-6 and bv <= 1e-9
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_columns_in_locality_order_solve_to_the_same_optimum(precision):
+    """``column_order="locality"`` solves the LP with its columns relabelled and
+    returns x and the reduced costs in the caller's order: same optimum as the
+    given order, x feasible for the original rows, d = c - A^T y."""
+    from sovlp.workloads import skewed_lp
+    print(f"[gpu] skewed 20k x 100k, locality order, precision {precision}", flush=True)
+    ko = skewed_lp(1, 20_000, 100_000, 800_000)
+    tol = 1e-8 if precision == 32 else 1e-6
+    par = PDLPParams(device="gpu", eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6,
+                     precision=precision, team="grid", column_order="locality")
+    s = solve_pdlp(ko.problem, par)
+    print(f"[gpu] locality done: {s.status.name} it={s.iterations}", flush=True)
+    assert s.status == Status.OPTIMAL
+    assert abs(s.objective - ko.f_star) <= tol * max(1.0, abs(ko.f_star))
+    rv, bv, _ = ko.problem.violation(s.x)
+    assert rv <= 1e-6 and bv <= 1e-9
+    np.testing.assert_allclose(s.reduced_costs, ko.problem.c - ko.problem.A.rmatvec(s.y),
+                               rtol=1e-9, atol=1e-9)
+    small = small_lp(0)
+    eps = dict(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6) if precision == 32 else {}
+    g = solve_pdlp(small.problem, PDLPParams(device="gpu", precision=precision, **eps))
+    o = solve_pdlp(small.problem, PDLPParams(device="gpu", precision=precision,
+                                             column_order="locality", **eps))
+    assert g.status == o.status == Status.OPTIMAL
+    assert abs(o.objective - g.objective) <= 1e-7 * max(1.0, abs(g.objective))

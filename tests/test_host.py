@@ -142,3 +142,52 @@ def test_column_panel_split_reproduces_the_matrix():
                 D[r, cols] += val[ptrs[p, r]:ptrs[p, r + 1]]
         np.testing.assert_array_equal(D, A.to_dense())
         assert ptrs[-1, -1] == A.nnz
+
+
+def test_locality_order_relabels_without_changing_any_sum():
+    from sovlp.core.sparse import locality_order, relabel_columns
+    from sovlp.workloads import skewed_lp_degrees
+    rng = np.random.default_rng(3)
+    m, n = 300, 2000
+    deg = skewed_lp_degrees(n, 12_000, dmax=200)[rng.permutation(n)]
+    cols = np.repeat(np.arange(n), deg)
+    A = SparseMatrix.from_triplets(rng.integers(0, m, cols.size), cols,
+                                   rng.standard_normal(cols.size), m, n)
+    new = locality_order(A, hot_degree=8)
+    assert np.array_equal(np.sort(new), np.arange(n))
+    cnt = np.diff(A.cp)
+    by_new = cnt[np.argsort(new)]
+    hot = by_new >= 8
+    assert hot[: hot.sum()].all() and np.all(np.diff(by_new[: hot.sum()]) <= 0)
+    first = A.ci[A.cp[:-1]][np.argsort(new)][hot.sum():]     # cold: by first row
+    assert np.all(np.diff(first[by_new[hot.sum():] > 0]) >= 0)
+    B = relabel_columns(A, new)
+    D = A.to_dense()
+    Dn = np.zeros_like(D)
+    Dn[:, new] = D
+    np.testing.assert_array_equal(B.to_dense(), Dn)
+    np.testing.assert_array_equal(B.rp, A.rp)
+    np.testing.assert_array_equal(B.rx, A.rx)              # rows keep their entry order
+    x = rng.standard_normal(n)
+    xn = np.empty(n)
+    xn[new] = x
+    for i in range(0, m, 7):                                 # row sums in storage order
+        a = 0.0
+        b = 0.0
+        for k in range(A.rp[i], A.rp[i + 1]):
+            a += A.rx[k] * x[A.ri[k]]
+            b += B.rx[k] * xn[B.ri[k]]
+        assert a == b
+    for j in range(0, n, 37):                                # columns move whole
+        q = new[j]
+        np.testing.assert_array_equal(B.ci[B.cp[q]:B.cp[q + 1]], A.ci[A.cp[j]:A.cp[j + 1]])
+        np.testing.assert_array_equal(B.cx[B.cp[q]:B.cp[q + 1]], A.cx[A.cp[j]:A.cp[j + 1]])
+    with pytest.raises(ValueError, match="permutation"):
+        relabel_columns(A, np.zeros(n, dtype=np.int64))
+
+
+def test_column_order_is_checked():
+    from sovlp.lp.pdlp import PDLPParams, solve_pdlp
+    from sovlp.workloads import small_lp
+    with pytest.raises(ValueError, match="column_order"):
+        solve_pdlp(small_lp(0).problem, PDLPParams(column_order="sorted"))
