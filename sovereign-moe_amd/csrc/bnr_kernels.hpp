@@ -102,9 +102,23 @@ __global__ void __launch_bounds__(kBnrBlock) bnr_start_kernel(BnrPtrs P) {
   P.X[q] = fmin(fmax(P.X[q], lo), hi);
 }
 
-// Xn = clip(X - tau (c + AtY), L, U), dX = Xn - X; sum dX^2 per node
-__global__ void __launch_bounds__(kBnrBlock) bnr_primal_kernel(BnrPtrs P) {
+// Xn = clip(X - tau (c + AtY), L, U), dX = Xn - X; sum dX^2 per node.  With
+// `pending`, the previous iteration's accepted nodes first take their trial
+// point (bnr_update_kernel, fused): X, AtY in the groups_n column blocks, Y, AX
+// in the blocks after them (launched only then; they write no group sums).
+__global__ void __launch_bounds__(kBnrBlock)
+    bnr_primal_kernel(BnrPtrs P, int pending, int groups_n) {
   __shared__ double lds[kBnrBlock];
+  if ((int)blockIdx.x >= groups_n) {
+    const int t = threadIdx.x;
+    const int kr = t & (P.KP - 1);
+    const long long r = (long long)((int)blockIdx.x - groups_n) * P.RB + t / P.KP;
+    if (kr >= P.K || r >= P.m || !P.accept[kr]) return;
+    const long long q = r * P.K + kr;
+    P.Y[q] = P.Yn[q];
+    P.AX[q] = P.AX[q] + P.AdX[q];
+    return;
+  }
   long long j;
   int k;
   double v[1] = {0.0};
@@ -112,8 +126,17 @@ __global__ void __launch_bounds__(kBnrBlock) bnr_primal_kernel(BnrPtrs P) {
 #pragma clang fp contract(off)
     const long long q = j * P.K + k;
     const double tau = P.eta[k] / P.omega[k];
-    const double x = P.X[q];
-    double xn = x - tau * (P.c[j] + P.AtY[q]);
+    double x, aty;
+    if (pending && P.accept[k]) {
+      x = P.Xn[q];
+      aty = P.AtY[q] + P.AtdY[q];
+      P.X[q] = x;
+      P.AtY[q] = aty;
+    } else {
+      x = P.X[q];
+      aty = P.AtY[q];
+    }
+    double xn = x - tau * (P.c[j] + aty);
     xn = fmin(fmax(xn, P.L[q]), P.U[q]);
     const double d = xn - x;
     P.Xn[q] = xn;
@@ -171,12 +194,30 @@ __global__ void __launch_bounds__(kBnrLanes)
     bnr_decide_kernel(BnrPtrs P, const double* part_p, int gp, const double* part_d, int gd,
                       double gfac, double dfac) {
 #pragma clang fp contract(off)
-  __shared__ double lds[kBnrLanes];
+  // the three sums in the order of bnr_groups_sum, with their loads issued
+  // together and their 256-term lane chains run side by side on lanes 0-2
+  __shared__ double lds[3 * kBnrLanes];
+  __shared__ double tot[3];
   const int k = blockIdx.x;
-  const double dxx = bnr_groups_sum(part_p, gp, 0, P.K, k, lds);
-  const double dyy = bnr_groups_sum(part_d, gd, 0, P.K, k, lds);
-  const double inter = bnr_groups_sum(part_d, gd, 1, P.K, k, lds);
-  if (threadIdx.x != 0) return;
+  const int t = threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int g = t; g < gp; g += kBnrLanes) a0 += part_p[(long long)g * 3 * P.K + k];
+  for (int g = t; g < gd; g += kBnrLanes) {
+    a1 += part_d[(long long)g * 3 * P.K + k];
+    a2 += part_d[((long long)g * 3 + 1) * P.K + k];
+  }
+  lds[t] = a0;
+  lds[kBnrLanes + t] = a1;
+  lds[2 * kBnrLanes + t] = a2;
+  __syncthreads();
+  if (t < 3) {
+    double sum = 0.0;
+    for (int i = 0; i < kBnrLanes; ++i) sum += lds[t * kBnrLanes + i];
+    tot[t] = sum;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  const double dxx = tot[0], dyy = tot[1], inter = tot[2];
   const double omega = P.omega[k], eta = P.eta[k];
   const double interaction = fabs(inter);
   const double movement = 0.5 * (omega * dxx + dyy / omega);

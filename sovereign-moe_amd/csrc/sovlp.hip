@@ -507,7 +507,10 @@ int sovlp_bnr_solve_batch(const sovlp_bnr_batch* b, void* workspace, int64_t wor
   Pd.part = part_d;
   Pb.part = part_b;
   for (int it = 0; it < b->iters; ++it) {
-    bnr_primal_kernel<<<gn, kBnrBlock, 0, st>>>(Pp);
+    // from the second iteration on, the previous iteration's update runs in
+    // the primal launch (its row part in gm extra blocks)
+    const int pending = it > 0 ? 1 : 0;
+    bnr_primal_kernel<<<gn + (pending && m > 0 ? gm : 0), kBnrBlock, 0, st>>>(Pp, pending, gn);
     if (m > 0) {
       bnr_spmm_kernel<<<gm, kBnrBlock, 0, st>>>(m, K, KP, RB, P.rp, P.ri, P.rx, P.dX, P.AdX);
       bnr_dual_kernel<<<gm, kBnrBlock, 0, st>>>(Pd);
@@ -517,9 +520,9 @@ int sovlp_bnr_solve_batch(const sovlp_bnr_batch* b, void* workspace, int64_t wor
     bnr_decide_kernel<<<K, kBnrLanes, 0, st>>>(P, part_p, gn, part_d, m > 0 ? gm : 0,
                                          1.0 - std::pow(t, -b->step_growth),
                                          1.0 + std::pow(t, -b->step_damp));
-    bnr_update_kernel<<<gn + (m > 0 ? gm : 0), kBnrBlock, 0, st>>>(P, gn);
     HIPCHK(hipGetLastError());
   }
+  if (b->iters > 0) bnr_update_kernel<<<gn + (m > 0 ? gm : 0), kBnrBlock, 0, st>>>(P, gn);
   // the bound from the final Y: A^T Y into AtdY, then the terms
   if (m > 0) {
     bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, KP, RB, P.cp, P.ci, P.cx, P.Y, P.AtdY);
