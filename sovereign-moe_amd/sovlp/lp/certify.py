@@ -83,7 +83,6 @@ class DeviceCertifier:
         self.ws = torch.empty(max(int(nb.value), 8), dtype=torch.uint8, device=dev)
         self.d = torch.empty(max(self.n, 1), dtype=torch.float64, device=dev)
         self.act = torch.empty(max(self.m, 1), dtype=torch.float64, device=dev)
-        self.stream = torch.cuda.current_stream(dev).cuda_stream
 
     def certify(self, x, y, dual_tol: float = 1e-6) -> _lib.sovlp_certificate:
         """The certificate of (x, y); x and y are host arrays or device tensors."""
@@ -102,7 +101,8 @@ class DeviceCertifier:
                   t["c"].data_ptr(), t["rl"].data_ptr(), t["ru"].data_ptr(), t["lo"].data_ptr(),
                   t["hi"].data_ptr(), self.offset, int(self.maximise), xs.data_ptr(),
                   ys.data_ptr(), float(dual_tol), self.ws.data_ptr(), self.ws.numel(),
-                  self.d.data_ptr(), self.act.data_ptr(), C.byref(out), self.stream)
+                  self.d.data_ptr(), self.act.data_ptr(), C.byref(out),
+                  torch.cuda.current_stream(xs.device).cuda_stream)   # where xs, ys were made
         return out
 
     def reduced_costs(self) -> np.ndarray:

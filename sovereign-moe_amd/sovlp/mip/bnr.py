@@ -79,7 +79,6 @@ class BNREngine:
                       ru=up(prob.row_ub, np.float64))
         self.obj_scale = getattr(scaling, "obj", 1.0) or 1.0
         self.obj_offset = prob.obj_offset
-        self.stream = torch.cuda.current_stream().cuda_stream
         self._alloc(self.cfg.batch)
         self.total_iterations = 0
         self.total_batches = 0
@@ -135,7 +134,9 @@ class BNREngine:
                                  omega.data_ptr(), float(self.cfg.step_growth),
                                  float(self.cfg.step_damp), bounds.data_ptr(),
                                  empty.data_ptr(), -1.0 if self.cfg.textbook_duals else 1.0)
-        _lib.call("sovlp_bnr_solve_batch", C.byref(b), ws.data_ptr(), ws.numel(), self.stream)
+        # the stream the uploads above were queued on
+        _lib.call("sovlp_bnr_solve_batch", C.byref(b), ws.data_ptr(), ws.numel(),
+                  torch.cuda.current_stream().cuda_stream)
         self.X[:, :K] = X
         self.Y[:, :K] = Y[: self.Y.shape[0]]
         self.eta[:K] = eta.cpu().numpy()

@@ -1,9 +1,10 @@
 """Batched node relaxation on the GPU against the oracle.
 
 The shared-A products are the oracle's bit for bit (same per-element order);
-the per-node sums (step-size test, bound) are fixed trees on the device and
-row-order sums in the oracle, so bounds and iterates agree to rounding
-(1e-9 relative), flags exactly.  Every bound is checked valid against HiGHS.
+the per-node sums (step-size test, bound) run in the device's fixed blocked
+order in both; bounds and iterates are compared to 1e-9 relative (the oracle's
+NumPy elementwise steps may round differently from the device's), flags
+exactly.  Every bound is checked valid against HiGHS.
 """
 
 import numpy as np
@@ -66,3 +67,27 @@ def test_a_model_without_rows():
         np.array([[1.0, 2.0], [0.0, 0.0]]), np.array([[5.0, 5.0], [3.0, 1.0]]))
     # min x0 - 2 x1: x0 at its lower bound, x1 at its upper bound
     np.testing.assert_array_equal(r.bounds, [1.0 - 6.0, 2.0 - 2.0])
+
+
+def test_calls_under_another_stream_match_the_default_stream():
+    """Engines built on the default stream and called under another stream queue
+    their uploads and kernels on that stream: the same results, bit for bit."""
+    import torch
+    from sovlp.lp.certify import DeviceCertifier
+    from sovlp.mip.bnr import BNRConfig, BNREngine
+    k = known_optimum_lp(30, 60, 3, seed=4, m_eq=15)
+    p = k.problem
+    L, U = node_boxes(p, 8, seed=3)
+    a = BNREngine(p, config=BNRConfig(batch=8, iters=60)).solve_batch(L, U)
+    eng = BNREngine(p, config=BNRConfig(batch=8, iters=60))
+    dc = DeviceCertifier(p)
+    ref = dc.certify(k.x_star, k.y_star)
+    ref = (ref.bound, ref.perturbation, ref.objective)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b = eng.solve_batch(L, U)
+        got = dc.certify(k.x_star, k.y_star)
+        got = (got.bound, got.perturbation, got.objective)
+    torch.cuda.synchronize()
+    assert np.array_equal(a.bounds, b.bounds) and np.array_equal(a.x, b.x)
+    assert got == ref
