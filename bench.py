@@ -223,10 +223,11 @@ def certify_rate(reps: int = 20) -> dict:
 
 
 def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int, tuned: bool,
-                     refine: dict | None):
+                     refine: dict | None, replays: int):
     """HBM bytes per launch of ``kernel`` on this workload from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's reading;
-    see profiles/), or None when they were taken on another configuration."""
+    see profiles/), or None when they were taken on another configuration or
+    with another number of graph replays per step."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_pdlp_run_kernel_fp32_4096_lps.json")
     try:
         d = json.load(open(path))
@@ -238,7 +239,8 @@ def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: in
     if refine:
         want += f", {refine['rounds']} rounds at {refine['round_eps']:g}, stall " \
                 f"{refine['stall_open']}/{refine['stall']}"
-    if d.get("kernel") != kernel or d.get("workload") != want:
+    if d.get("kernel") != kernel or d.get("workload") != want \
+            or d.get("replays_per_step") != replays:
         return None
     return float(d["traffic_bytes_per_launch"])
 
@@ -434,7 +436,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(kernel, a.lps, a.seed, a.precision, a.block,
                                                  tuner is not None,
-                                                 refinement_config(par, a.precision)),
+                                                 refinement_config(par, a.precision),
+                                                 int(graphs[-1])),
                      "algorithmic_bytes_per_launch": step_bytes / graphs[-1],
                      "avg_launch_us": launch_ms * 1e3, "launches": launches},
     }

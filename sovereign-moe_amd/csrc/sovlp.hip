@@ -830,7 +830,11 @@ void launch_run(int block, const LpDev<T>* lps, int count, const StepCtl* ctl, u
   else
     pdlp_run_kernel<T, 256, T256><<<count, 256, 0, st>>>(lps, ctl, done, phases);
 }
-constexpr int kPhasesPerLaunch = 1024;  // run kernel: phases per LP per launch (128: 2.9 % slower)
+// run kernel: phases per LP per launch. Every launch ends when its slowest LP
+// has run its phases, so shorter launches leave CUs idle at each end (4096
+// uneven LPs: 1024 phases 8470 LPs/s, 16384 11170, 65536 12012, 262144 12101);
+// time and iteration limits are enforced on the device, not by this count.
+constexpr int kPhasesPerLaunch = 65536;
 
 template <class T>
 void fill_lp(LpDev<T>& L, const sovlp_lp& lp) {
