@@ -355,7 +355,7 @@ int sovlp_batch_workspace_bytes(int count, const sovlp_lp *lps, int precision,
 
 /* Lays out the batch in the workspace and records one replay into a
  * hipGraph: one launch, one workgroup of `block` threads per LP, each running
- * up to 128 phases of its LP back to back.  `block` is 64, 128 or 256.
+ * up to 65536 phases of its LP back to back.  `block` is 64, 128 or 256.
  * Synchronises. */
 int sovlp_batch_create(int count, const sovlp_lp *lps, const sovlp_params *params,
                        int precision, int block, void *workspace, int64_t workspace_bytes,
