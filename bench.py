@@ -305,7 +305,7 @@ def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every LP of the batch lives on one device and "
-              "this build runs no multi-GPU path (--gpus must be 1).", file=sys.stderr)
+              "sovlp has no multi-GPU path (--gpus must be 1).", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

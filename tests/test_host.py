@@ -191,3 +191,31 @@ def test_column_order_is_checked():
     from sovlp.workloads import small_lp
     with pytest.raises(ValueError, match="column_order"):
         solve_pdlp(small_lp(0).problem, PDLPParams(column_order="sorted"))
+
+
+def test_bench_reports_traffic_only_for_the_measured_configuration():
+    """bench.py reads the committed PMC traffic only for the workload, kernel and
+    replays per step it was measured on."""
+    import json
+    import bench
+    from sovlp.lp.pdlp import PDLPParams
+    d = json.load(open(os.path.join(bench.ROOT, "profiles",
+                                    "pmc_traffic_pdlp_run_kernel_fp32_4096_lps.json")))
+    par = PDLPParams(refine_stall_open=1024)
+    refine = bench.refinement_config(par, 32)
+    args = (d["kernel"], 4096, 1000, 32, 256, True, refine)
+    assert bench.measured_traffic(*args, d["replays_per_step"]) == d["traffic_bytes_per_launch"]
+    assert bench.measured_traffic(*args, d["replays_per_step"] + 25) is None
+    assert bench.measured_traffic(d["kernel"], 4096, 1000, 32, 256, False, refine,
+                                  d["replays_per_step"]) is None
+    assert bench.measured_traffic(*args[:6], bench.refinement_config(PDLPParams(), 32),
+                                  d["replays_per_step"]) is None
+    assert 0.5 < d["traffic_bytes_per_launch"] / d["algorithmic_bytes_per_launch"] < 1.5
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(os.path.dirname(__file__), "..", "bench.py"),
+                        "--gpus", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
