@@ -14,7 +14,6 @@ used so the gathered vector is not trivially cache-resident between launches.
 
 from __future__ import annotations
 
-import ctypes as C
 import time
 
 import numpy as np
