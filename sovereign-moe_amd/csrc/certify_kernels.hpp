@@ -130,6 +130,88 @@ __global__ void __launch_bounds__(kCertBlock)
   }
 }
 
+// The same list without atomics, in index order, in three launches over G
+// workgroups, workgroup g taking lines [g chunk, (g + 1) chunk): the long
+// lines of each chunk counted, the counts scanned by one workgroup (offsets in
+// place, the total into *total), each chunk's lines written from its offset.
+// (Same-address atomics are serialised beyond the per-XCD L2: a few hundred
+// long lines cost tens of microseconds in long_list_kernel.)
+__device__ __forceinline__ long long chunk_end(int lines, int chunk, long long b0) {
+  const long long e = b0 + chunk;
+  return e < lines ? e : (long long)lines;
+}
+
+__global__ void __launch_bounds__(kCertBlock)
+    long_count_kernel(int lines, int chunk, const int64_t* __restrict__ ptr, int* counts) {
+  __shared__ int wsum[kCertBlock / kWave];
+  const long long b0 = (long long)blockIdx.x * chunk, b1 = chunk_end(lines, chunk, b0);
+  int c = 0;
+  for (long long i = b0 + threadIdx.x; i < b1; i += kCertBlock) c += long_line(ptr, i) ? 1 : 0;
+  for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kCertBlock / kWave; ++w) t += wsum[w];
+    counts[blockIdx.x] = t;
+  }
+}
+
+// groups <= 4 * kCertBlock; one workgroup
+__global__ void __launch_bounds__(kCertBlock)
+    long_scan_kernel(int groups, int* counts, int* total) {
+  __shared__ int sc[kCertBlock];
+  const int t = (int)threadIdx.x;
+  int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int g = 4 * t + k;
+    v[k] = g < groups ? counts[g] : 0;
+    s += v[k];
+  }
+  sc[t] = s;
+  __syncthreads();
+  for (int o = 1; o < kCertBlock; o <<= 1) {
+    const int add = t >= o ? sc[t - o] : 0;
+    __syncthreads();
+    sc[t] += add;
+    __syncthreads();
+  }
+  int base = sc[t] - s;                    // exclusive prefix of thread t's four counts
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int g = 4 * t + k;
+    if (g < groups) counts[g] = base;
+    base += v[k];
+  }
+  if (t == kCertBlock - 1) *total = sc[t];
+}
+
+__global__ void __launch_bounds__(kCertBlock)
+    long_write_kernel(int lines, int chunk, const int64_t* __restrict__ ptr,
+                      const int* __restrict__ offsets, int* list) {
+  __shared__ int wcnt[kCertBlock / kWave];
+  const long long b0 = (long long)blockIdx.x * chunk, b1 = chunk_end(lines, chunk, b0);
+  const int lane = (int)threadIdx.x & (kWave - 1), w = (int)threadIdx.x / kWave;
+  int base = offsets[blockIdx.x];
+  for (long long i0 = b0; i0 < b1; i0 += kCertBlock) {
+    const long long i = i0 + threadIdx.x;
+    const bool lng = i < b1 && long_line(ptr, i);
+    const unsigned long long mask = __ballot(lng);
+    if (lane == 0) wcnt[w] = (int)__popcll(mask);
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int u = 0; u < kCertBlock / kWave; ++u) {
+      before += u < w ? wcnt[u] : 0;
+      all += wcnt[u];
+    }
+    const int pos = base + before + (int)__popcll(mask & ((1ull << lane) - 1ull));
+    if (lng && pos < lines) list[pos] = (int)i;
+    base += all;
+    __syncthreads();                     // wcnt is rewritten by the next round
+  }
+}
+
 __global__ void zero_ints_kernel(int* p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }

@@ -362,7 +362,7 @@ int sovlp_certify_workspace_bytes(int m, int n, int64_t* bytes) {
   if (!bytes) return fail(1, "null argument");
   if (m < 0 || n < 0) return fail(1, "negative size");
   *bytes = 8 * (2 * (int64_t)m + n + 15 * (int64_t)kCertMaxGroups + 8) +
-           4 * ((int64_t)m + n + 4);
+           4 * ((int64_t)m + n + 4 + 2 * (int64_t)kCertMaxGroups);
   return 0;
 }
 
@@ -396,14 +396,27 @@ int sovlp_certify_f64(int m, int n, const int64_t* rp, const int32_t* ri, const 
   int* cnt = reinterpret_cast<int*>(P.res + 8);     // two counters, then the lists
   int* list_r = cnt + 4;
   int* list_c = list_r + m;
+  int* counts_r = list_c + n;                       // per-chunk counts, then offsets
+  int* counts_c = counts_r + kCertMaxGroups;
   const int g1 = cert_groups(m), g2 = cert_groups(n);
   hipStream_t st = S(stream);
+  // lists of the lines longer than kLongLine: chunks of the lines over at most
+  // kCertMaxGroups workgroups (the scan takes up to 4 counts per thread)
+  static_assert(kCertMaxGroups <= 4 * kCertBlock, "long_scan_kernel covers 4 counts per thread");
+  auto list_long = [&](int lines, const int64_t* ptr, int* counts, int* total, int* list) -> int {
+    const int groups = std::max(1, std::min(kCertMaxGroups, (lines + kCertBlock - 1) / kCertBlock));
+    const int chunk = (int)(((int64_t)lines + groups - 1) / groups);
+    long_count_kernel<<<groups, kCertBlock, 0, st>>>(lines, chunk, ptr, counts);
+    long_scan_kernel<<<1, kCertBlock, 0, st>>>(groups, counts, total);
+    long_write_kernel<<<groups, kCertBlock, 0, st>>>(lines, chunk, ptr, counts, list);
+    HIPCHK(hipGetLastError());
+    return 0;
+  };
   // long rows' activities, then the rows (yy), long columns' d, the columns
   zero_ints_kernel<<<1, 64, 0, st>>>(cnt, 2);
   HIPCHK(hipGetLastError());
   if (m > 0) {
-    long_list_kernel<<<g1, kCertBlock, 0, st>>>(m, rp, cnt, list_r);
-    HIPCHK(hipGetLastError());
+    if (int rc = list_long(m, rp, counts_r, cnt, list_r)) return rc;
     long_lines_kernel<true><<<kCertMaxGroups, kCertBlock, 0, st>>>(cnt, list_r, rp, ri, rx, x,
                                                                      nullptr, -1.0, P.act);
     HIPCHK(hipGetLastError());
@@ -411,9 +424,7 @@ int sovlp_certify_f64(int m, int n, const int64_t* rp, const int32_t* ri, const 
   cert_rows_kernel<<<g1, kCertBlock, 0, st>>>(P);
   HIPCHK(hipGetLastError());
   if (n > 0) {
-    long_list_kernel<<<std::min(4 * kCertMaxGroups, (n + kCertBlock - 1) / kCertBlock),
-                       kCertBlock, 0, st>>>(n, cp, cnt + 1, list_c);
-    HIPCHK(hipGetLastError());
+    if (int rc = list_long(n, cp, counts_c, cnt + 1, list_c)) return rc;
     long_lines_kernel<false><<<kCertMaxGroups, kCertBlock, 0, st>>>(
         cnt + 1, list_c, cp, ci, cx, P.yy, c, maximise ? -1.0 : 1.0, P.d);
     HIPCHK(hipGetLastError());
