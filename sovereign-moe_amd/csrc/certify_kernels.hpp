@@ -111,31 +111,14 @@ __global__ void __launch_bounds__(kCertBlock)
     if (!long_line(rp, i)) out[i] = compensated_row(rp, ri, rx, x, b[i], i);
 }
 
-// The lines longer than kLongLine, listed (in any order: each line's result
+// The lines longer than kLongLine, listed in index order (each line's result
 // depends on the line alone) so that the long-line kernel can deal them out
-// one per workgroup.  count must start at 0.
-__global__ void __launch_bounds__(kCertBlock)
-    long_list_kernel(int lines, const int64_t* __restrict__ ptr, int* count, int* list) {
-  const long long stride = (long long)gridDim.x * kCertBlock;
-  const int lane = (int)threadIdx.x % kWave;
-  // one atomic per wave (long lines cluster, and atomics on one address serialise)
-  for (long long i0 = (long long)blockIdx.x * kCertBlock; i0 < lines; i0 += stride) {
-    const long long i = i0 + threadIdx.x;
-    const bool lng = i < lines && long_line(ptr, i);
-    const unsigned long long mask = __ballot(lng);
-    int base = 0;
-    if (lane == 0 && mask != 0ull) base = atomicAdd(count, (int)__popcll(mask));
-    base = __shfl(base, 0);
-    if (lng) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
-  }
-}
-
-// The same list without atomics, in index order, in three launches over G
-// workgroups, workgroup g taking lines [g chunk, (g + 1) chunk): the long
-// lines of each chunk counted, the counts scanned by one workgroup (offsets in
-// place, the total into *total), each chunk's lines written from its offset.
-// (Same-address atomics are serialised beyond the per-XCD L2: a few hundred
-// long lines cost tens of microseconds in long_list_kernel.)
+// one per workgroup.  Three launches over G workgroups, workgroup g taking
+// lines [g chunk, (g + 1) chunk): the long lines of each chunk counted, the
+// counts scanned by one workgroup (offsets in place, the total into *total),
+// each chunk's lines written from its offset.  No atomics: same-address
+// atomics serialise beyond the per-XCD L2, and a wave-aggregated atomic list
+// of 331 long columns took 55 us.
 __device__ __forceinline__ long long chunk_end(int lines, int chunk, long long b0) {
   const long long e = b0 + chunk;
   return e < lines ? e : (long long)lines;

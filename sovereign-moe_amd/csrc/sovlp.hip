@@ -326,7 +326,22 @@ int sovlp_spmv_csr_f32(int rows, const int64_t* rp, const int32_t* ri, const flo
 int sovlp_residual_workspace_bytes(int rows, int64_t* bytes) {
   if (!bytes) return fail(1, "null argument");
   if (rows < 0) return fail(1, "negative row count");
-  *bytes = 4 * ((int64_t)rows + 4);
+  *bytes = 4 * ((int64_t)rows + 4 + kCertMaxGroups);
+  return 0;
+}
+
+// The lines of more than kLongLine nonzeros, in index order, into list[0..*total):
+// chunks of the lines over at most kCertMaxGroups workgroups, counted, scanned
+// (counts[kCertMaxGroups] holds the counts, then the offsets) and written.
+static_assert(kCertMaxGroups <= 4 * kCertBlock, "long_scan_kernel covers 4 counts per thread");
+static int list_long_lines(int lines, const int64_t* ptr, int* counts, int* total, int* list,
+                           hipStream_t st) {
+  const int groups = std::max(1, std::min(kCertMaxGroups, (lines + kCertBlock - 1) / kCertBlock));
+  const int chunk = (int)(((int64_t)lines + groups - 1) / groups);
+  long_count_kernel<<<groups, kCertBlock, 0, st>>>(lines, chunk, ptr, counts);
+  long_scan_kernel<<<1, kCertBlock, 0, st>>>(groups, counts, total);
+  long_write_kernel<<<groups, kCertBlock, 0, st>>>(lines, chunk, ptr, counts, list);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 
@@ -336,16 +351,15 @@ int sovlp_residual_f64(int rows, const int64_t* rp, const int32_t* ri, const dou
   if (rows < 0) return fail(1, "negative row count");
   if (rows == 0) return 0;
   if (!rp || !ri || !rx || !x || !b || !out || !workspace) return fail(1, "null argument");
-  if (workspace_bytes < 4 * ((int64_t)rows + 4))
-    return fail(1, "workspace too small (sovlp_residual_workspace_bytes)");
+  int64_t need = 0;
+  sovlp_residual_workspace_bytes(rows, &need);
+  if (workspace_bytes < need) return fail(1, "workspace too small (sovlp_residual_workspace_bytes)");
   if (reinterpret_cast<uintptr_t>(workspace) % 4) return fail(1, "workspace not 4-byte aligned");
-  int* cnt = static_cast<int*>(workspace);   // the count of long rows, then their list
+  // the count of long rows, their list, then the listing's per-chunk counts
+  int* cnt = static_cast<int*>(workspace);
   hipStream_t st = S(stream);
   const int groups = std::min(kCertMaxGroups * 4, (rows + kCertBlock - 1) / kCertBlock);
-  zero_ints_kernel<<<1, 64, 0, st>>>(cnt, 1);
-  HIPCHK(hipGetLastError());
-  long_list_kernel<<<groups, kCertBlock, 0, st>>>(rows, rp, cnt, cnt + 4);
-  HIPCHK(hipGetLastError());
+  if (int rc = list_long_lines(rows, rp, cnt + 4 + rows, cnt, cnt + 4, st)) return rc;
   residual_kernel<<<groups, kCertBlock, 0, st>>>(rows, rp, ri, rx, x, b, out);
   HIPCHK(hipGetLastError());
   long_lines_kernel<true><<<kCertMaxGroups, kCertBlock, 0, st>>>(cnt, cnt + 4, rp, ri, rx, x, b,
@@ -400,23 +414,11 @@ int sovlp_certify_f64(int m, int n, const int64_t* rp, const int32_t* ri, const 
   int* counts_c = counts_r + kCertMaxGroups;
   const int g1 = cert_groups(m), g2 = cert_groups(n);
   hipStream_t st = S(stream);
-  // lists of the lines longer than kLongLine: chunks of the lines over at most
-  // kCertMaxGroups workgroups (the scan takes up to 4 counts per thread)
-  static_assert(kCertMaxGroups <= 4 * kCertBlock, "long_scan_kernel covers 4 counts per thread");
-  auto list_long = [&](int lines, const int64_t* ptr, int* counts, int* total, int* list) -> int {
-    const int groups = std::max(1, std::min(kCertMaxGroups, (lines + kCertBlock - 1) / kCertBlock));
-    const int chunk = (int)(((int64_t)lines + groups - 1) / groups);
-    long_count_kernel<<<groups, kCertBlock, 0, st>>>(lines, chunk, ptr, counts);
-    long_scan_kernel<<<1, kCertBlock, 0, st>>>(groups, counts, total);
-    long_write_kernel<<<groups, kCertBlock, 0, st>>>(lines, chunk, ptr, counts, list);
-    HIPCHK(hipGetLastError());
-    return 0;
-  };
   // long rows' activities, then the rows (yy), long columns' d, the columns
   zero_ints_kernel<<<1, 64, 0, st>>>(cnt, 2);
   HIPCHK(hipGetLastError());
   if (m > 0) {
-    if (int rc = list_long(m, rp, counts_r, cnt, list_r)) return rc;
+    if (int rc = list_long_lines(m, rp, counts_r, cnt, list_r, st)) return rc;
     long_lines_kernel<true><<<kCertMaxGroups, kCertBlock, 0, st>>>(cnt, list_r, rp, ri, rx, x,
                                                                      nullptr, -1.0, P.act);
     HIPCHK(hipGetLastError());
@@ -424,7 +426,7 @@ int sovlp_certify_f64(int m, int n, const int64_t* rp, const int32_t* ri, const 
   cert_rows_kernel<<<g1, kCertBlock, 0, st>>>(P);
   HIPCHK(hipGetLastError());
   if (n > 0) {
-    if (int rc = list_long(n, cp, counts_c, cnt + 1, list_c)) return rc;
+    if (int rc = list_long_lines(n, cp, counts_c, cnt + 1, list_c, st)) return rc;
     long_lines_kernel<false><<<kCertMaxGroups, kCertBlock, 0, st>>>(
         cnt + 1, list_c, cp, ci, cx, P.yy, c, maximise ? -1.0 : 1.0, P.d);
     HIPCHK(hipGetLastError());
