@@ -21,11 +21,17 @@ Beside the headline number the JSON line carries:
                 more LPs finish, in more time
   roofline      algorithmic bytes of the step kernel (phase counts x the byte
                 model of each phase) / its average launch time (HIP events on
-                the launch stream), against 8 TB/s
+                the launch stream), against 8 TB/s; `traffic` the HBM bytes per
+                launch from the committed FETCH_SIZE / WRITE_SIZE passes when
+                they were taken on this workload and launch structure
   cpu_baseline  the FP64 C restatement of the reference solver on a bounded
                 sample of the batch, one LP per host thread
-  spmv          the tile-pass SpMV of A and A^T of the 200k x 1M skewed LP
-                (optional, --spmv)
+  spmv          the tile-pass SpMV of A and A^T of the 200k x 1M skewed LP, in
+                the given and the locality column order (optional, --spmv)
+  bnr           batched node relaxation, 128 nodes of a 500 x 2000 LP
+                (optional, --bnr)
+  certify       the certificate of a point of the 200k x 1M LP (optional,
+                --certify)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lps 4096]
                        [--precision 64|32] [--block 256|64]
