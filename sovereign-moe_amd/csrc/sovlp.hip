@@ -1,30 +1,17 @@
-// sovlp.hip -- kernels and C ABI of libsovlp.so (gfx950).
+// sovlp.hip -- Backend surface, tiled SpMV, certificate and batched node
+// relaxation kernels, and their C ABI (gfx950).  The PDLP engines are in
+// engine.hpp (compiled per precision by engine_inst.hip) and engine_api.hip.
 //
 // Kernels:
 //   spmv_vector_kernel   Backend surface SpMV, TPR lanes per row
 //   elementwise kernels  Backend surface steps (backend.py:204-291)
 //   reduce_kernel        Backend surface dot / nrm2 / nrm_inf, fixed order
 //   spmv_tiled_kernel    the engines' CSR-Adaptive tile pass as a bare SpMV
-//   pdlp_run_kernel      PDLP phases of each LP, one workgroup per LP
-//   pdlp_phase_kernel    one PDLP phase of one LP over W workgroups
-//   pdlp_decide_kernel   reduces the W partial sums and takes the transition
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "../../include/sovlp.h"
-#include "pdlp_kernels.hpp"
+#include "common.hpp"
 #include "certify_kernels.hpp"
 #include "bnr_kernels.hpp"
 
-using namespace sov;
-
-namespace {
+namespace sovh {
 
 thread_local std::string g_err;
 
@@ -33,18 +20,37 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                     \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess)                                                                \
-      return fail(2, std::string(#expr) + ": " + hipGetErrorString(e_));                \
-  } while (0)
+int clock_hz(double* hz) {
+  int dev = 0, khz = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  *hz = (double)khz * 1000.0;
+  return 0;
+}
 
-inline hipStream_t S(sovlp_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+void host_partition(int ntiles, const int32_t* tiles, const int64_t* ptr_at_tile, int parts,
+                    int32_t* part) {
+  // ptr_at_tile[t] = pointer value at row tiles[t]; cost = nonzeros + rows
+  std::vector<double> pre(ntiles + 1, 0.0);
+  for (int t = 0; t < ntiles; ++t)
+    pre[t + 1] = pre[t] + (double)(ptr_at_tile[t + 1] - ptr_at_tile[t]) +
+                 (double)(tiles[t + 1] - tiles[t]);
+  const double total = pre[ntiles];
+  part[0] = 0;
+  int t = 0;
+  for (int p = 1; p < parts; ++p) {
+    const double target = total * (double)p / (double)parts;
+    while (t < ntiles && pre[t] < target) ++t;
+    part[p] = t;
+  }
+  part[parts] = ntiles;
+}
 
-constexpr int kBlock = 256;
-constexpr int kTnnz64 = 2048;   // FP64 engines: 2 vectors x 2048 x 8 B = 32 KiB LDS
-constexpr int kTnnz32 = 4096;   // FP32 engines: 2 x 4096 x 4 B (or 4096 doubles) = 32 KiB
+}  // namespace sovh
+
+using namespace sovh;
+
+namespace {
 
 // ---------------------------------------------------------------- Backend
 template <class T, int TPR>
@@ -214,87 +220,12 @@ __global__ void panel_sum_kernel(int panels, int rows, const T* __restrict__ par
   y[r] = s;
 }
 
-// ---------------------------------------------------------------- workspace
-struct Carver {
-  char* base;
-  size_t off;
-  size_t cap;
-  template <class U>
-  U* take(size_t count) {
-    off = (off + 255) & ~size_t(255);
-    U* p = reinterpret_cast<U*>(base ? base + off : nullptr);
-    off += sizeof(U) * (count ? count : 1);
-    return p;
-  }
-};
-
-template <class T>
-void carve_lp(Carver& cv, LpDev<T>& L, int m, int n, long long nnz, long long hist_cap) {
-  L.sav = cv.take<T>(nnz);
-  L.stv = cv.take<T>(nnz);
-  L.rowf = cv.take<double>(m);
-  L.colf = cv.take<double>(n);
-  L.tmpm = cv.take<double>(m);
-  L.tmpn = cv.take<double>(n);
-  T** cols[] = {&L.c, &L.lo, &L.hi, &L.x, &L.xn, &L.dx, &L.aty, &L.sumx, &L.xr, &L.atyf, &L.atya};
-  for (T** p : cols) *p = cv.take<T>(n);
-  T** rows[] = {&L.rl, &L.ru, &L.y, &L.yn, &L.dy, &L.ax, &L.adx, &L.sumy, &L.yr, &L.axf, &L.axa};
-  for (T** p : rows) *p = cv.take<T>(m);
-  L.xo = cv.take<double>(n);
-  L.yo = cv.take<double>(m);
-  L.dout = cv.take<double>(n);
-  L.hist = cv.take<double>(5 * (size_t)(hist_cap > 0 ? hist_cap : 1));
-  L.hist_cap = hist_cap;
-  L.rhist = cv.take<double>(5 * kMaxRounds);
-  L.st = cv.take<LpState>(1);
-  if (sizeof(T) == 4) {      // FP32 iterates: room for iterative refinement
-    L.wrl = cv.take<double>(m);
-    L.wru = cv.take<double>(m);
-    L.wlo = cv.take<double>(n);
-    L.whi = cv.take<double>(n);
-    L.xacc = cv.take<double>(n);
-    L.axacc = cv.take<double>(m);
-    L.yacc = cv.take<double>(m);
-    L.qc = cv.take<double>(m);
-    L.dacc = cv.take<double>(n);
-    L.cc = cv.take<double>(n);
-    L.q = reinterpret_cast<T*>(cv.take<float>(m));
-    L.xbest = cv.take<double>(n);
-    L.ybest = cv.take<double>(m);
-  }
-}
-
-int clock_hz(double* hz) {
-  int dev = 0, khz = 0;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-  *hz = (double)khz * 1000.0;
-  return 0;
-}
-
-void host_partition(int ntiles, const int32_t* tiles, const int64_t* ptr_at_tile, int parts,
-                    int32_t* part) {
-  // ptr_at_tile[t] = pointer value at row tiles[t]; cost = nonzeros + rows
-  std::vector<double> pre(ntiles + 1, 0.0);
-  for (int t = 0; t < ntiles; ++t)
-    pre[t + 1] = pre[t] + (double)(ptr_at_tile[t + 1] - ptr_at_tile[t]) +
-                 (double)(tiles[t + 1] - tiles[t]);
-  const double total = pre[ntiles];
-  part[0] = 0;
-  int t = 0;
-  for (int p = 1; p < parts; ++p) {
-    const double target = total * (double)p / (double)parts;
-    while (t < ntiles && pre[t] < target) ++t;
-    part[p] = t;
-  }
-  part[parts] = ntiles;
-}
 
 }  // namespace
 
 extern "C" {
 
-const char* sovlp_last_error(void) { return g_err.c_str(); }
+const char* sovlp_last_error(void) { return sovh::g_err.c_str(); }
 
 int sovlp_version(void) { return 100; }
 
@@ -777,1009 +708,3 @@ int sovlp_selftest(void) {
 
 }  // extern "C"
 
-// ====================================================================== engines
-struct sovlp_pdlp {
-  int precision, W, m, n;
-  void* lpdev;               // LpDev<T> (device)
-  LpState* st;
-  StepCtl* ctl;
-  unsigned* done;
-  double *xo, *yo, *dout, *hist, *rhist;
-  long long* pcount;         // [kPhaseSlots]
-  int phases_per_replay;
-  long long hist_cap;
-  double hz;
-  bool started;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-  hipStream_t cap;
-  long long ticks_set;
-  LpState* hstate;           // [2] pinned host copies of the state
-  hipEvent_t ev[2];
-};
-
-struct sovlp_batch {
-  int count, precision, block;
-  void* lps;                 // LpDev<T>[count] (device)
-  StepCtl* ctl;
-  unsigned* done;
-  long long* pcount;         // [count][kPhaseSlots]
-  std::vector<double*> xo, yo, dout, rhist;
-  std::vector<LpState*> st;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-  hipStream_t cap;
-  double hz;
-  long long ticks_set;
-  unsigned* hdone;           // [2] pinned host copies of the done counter
-  hipEvent_t ev[2];
-  hipStream_t side[2];       // capture branches of the second and third groups
-  hipEvent_t fork, join[2];
-};
-
-namespace {
-
-constexpr int kTnnzSmall = 512;   // 64-thread workgroups: 2 x 512 x 8 B = 8 KiB LDS
-constexpr int kPairsPerGraph = 64;     // W > 1: phase + decide launches per graph
-
-// Workgroup sizes of the batched engine, and their tile nonzero caps (16 KiB
-// of LDS for 128 threads, 32 KiB for 256)
-bool batch_block_ok(int block) { return block == 64 || block == 128 || block == 256; }
-int batch_tnnz(int precision, int block) {
-  if (block == 64) return kTnnzSmall;
-  if (block == 128) return precision == 64 ? kTnnz64 / 2 : kTnnz32 / 2;
-  return precision == 64 ? kTnnz64 : kTnnz32;
-}
-
-// One launch of the run kernel over `count` LP descriptors with `block` threads each
-template <class T>
-void launch_run(int block, const LpDev<T>* lps, int count, const StepCtl* ctl, unsigned* done,
-                int phases, hipStream_t st) {
-  constexpr int T256 = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
-  if (block == 64)
-    pdlp_run_kernel<T, 64, kTnnzSmall><<<count, 64, 0, st>>>(lps, ctl, done, phases);
-  else if (block == 128)
-    pdlp_run_kernel<T, 128, T256 / 2><<<count, 128, 0, st>>>(lps, ctl, done, phases);
-  else
-    pdlp_run_kernel<T, 256, T256><<<count, 256, 0, st>>>(lps, ctl, done, phases);
-}
-// run kernel: phases per LP per launch. Every launch ends when its slowest LP
-// has run its phases, so shorter launches leave CUs idle at each end (4096
-// uneven LPs: 1024 phases 8470 LPs/s, 16384 11170, 65536 12012, 262144 12101);
-// time and iteration limits are enforced on the device, not by this count.
-constexpr int kPhasesPerLaunch = 65536;
-
-template <class T>
-void fill_lp(LpDev<T>& L, const sovlp_lp& lp) {
-  L.m = lp.m; L.n = lp.n; L.ntA = lp.ntiles_a; L.ntT = lp.ntiles_t;
-  L.flip = lp.maximise ? 1 : 0;
-  L.offset0 = lp.obj_offset;
-  L.ap = lp.rp; L.ai = lp.ri; L.av = lp.rx;
-  L.tp = lp.cp; L.ti = lp.ci; L.tv = lp.cx;
-  L.c0 = lp.c; L.lo0 = lp.lo; L.hi0 = lp.hi; L.rl0 = lp.rl; L.ru0 = lp.ru; L.v0 = lp.v0;
-  L.tilesA = lp.tiles_a;
-  L.tilesT = lp.tiles_t;
-  if (L.xacc) {              // the rounds' bounds live in the workspace
-    L.crl = L.wrl; L.cru = L.wru; L.clo = L.wlo; L.chi = L.whi;
-  } else {                   // no refinement: the original bounds
-    L.crl = lp.rl; L.cru = lp.ru; L.clo = lp.lo; L.chi = lp.hi;
-  }
-}
-
-// workgroups sharing one LP's phases: one per ~4 tiles, at most what is resident
-template <class T>
-int pick_width(const sovlp_lp* lp, int team, int* W) {
-  int dev = 0, cus = 0, nb = 0;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, pdlp_phase_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>, kBlock, 0));
-  const long long tiles = (long long)lp->ntiles_a + lp->ntiles_t;
-  const long long resident = (long long)std::max(nb, 1) * cus;
-  long long w = std::max<long long>(1, tiles / 4);
-  if (team == 1) w = 1;
-  if (team == 0 && lp->nnz < 200000) w = 1;
-  *W = (int)std::min(w, resident);
-  return 0;
-}
-
-template <class T>
-size_t single_ws_T(const sovlp_lp* lp, long long hist_cap, int W) {
-  Carver cv{nullptr, 0, 0};
-  cv.take<LpDev<T>>(1);
-  cv.take<StepCtl>(1);
-  cv.take<unsigned>(64);
-  cv.take<double>((size_t)W * kMaxRed);
-  cv.take<int32_t>(W + 1);
-  cv.take<int32_t>(W + 1);
-  cv.take<long long>(kPhaseSlots);
-  LpDev<T> L{};
-  carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
-  return cv.off + 256;
-}
-
-template <class T>
-int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64_t ws_bytes,
-                    int64_t hist_cap, int team, hipStream_t st, sovlp_pdlp* h) {
-  int W = 1;
-  int rc = pick_width<T>(lp, team, &W);
-  if (rc) return rc;
-  if ((size_t)ws_bytes < single_ws_T<T>(lp, hist_cap, W)) return fail(1, "workspace too small");
-  Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
-  LpDev<T>* d_lp = cv.take<LpDev<T>>(1);
-  StepCtl* ctl = cv.take<StepCtl>(1);
-  unsigned* done = cv.take<unsigned>(64);
-  double* partials = cv.take<double>((size_t)W * kMaxRed);
-  int32_t* partA = cv.take<int32_t>(W + 1);
-  int32_t* partT = cv.take<int32_t>(W + 1);
-  long long* pcount = cv.take<long long>(kPhaseSlots);
-  LpDev<T> L{};
-  carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
-  fill_lp<T>(L, *lp);
-  L.partials = partials;
-  L.pcount = pcount;
-  if (W > 1) {
-    // equal-cost split of both tile lists (needs the pointer values at tile starts)
-    std::vector<int32_t> ta(lp->ntiles_a + 1), tt(lp->ntiles_t + 1), rp(lp->m + 1), cp(lp->n + 1);
-    HIPCHK(hipMemcpyAsync(ta.data(), lp->tiles_a, sizeof(int32_t) * ta.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(tt.data(), lp->tiles_t, sizeof(int32_t) * tt.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(rp.data(), lp->rp, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(cp.data(), lp->cp, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<int64_t> pa(ta.size()), pt(tt.size());
-    for (size_t t = 0; t < ta.size(); ++t) pa[t] = rp[ta[t]];
-    for (size_t t = 0; t < tt.size(); ++t) pt[t] = cp[tt[t]];
-    std::vector<int32_t> ha(W + 1), ht(W + 1);
-    host_partition(lp->ntiles_a, ta.data(), pa.data(), W, ha.data());
-    host_partition(lp->ntiles_t, tt.data(), pt.data(), W, ht.data());
-    HIPCHK(hipMemcpyAsync(partA, ha.data(), sizeof(int32_t) * (W + 1), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(partT, ht.data(), sizeof(int32_t) * (W + 1), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-    L.partA = partA;
-    L.partT = partT;
-  }
-  LpState s0{};
-  s0.phase = PH_INIT;
-  s0.status = ST_RUNNING;
-  StepCtl c0{};
-  std::memcpy(&c0.P, prm, sizeof(Params));
-  c0.ticks = (long long)9.0e18;
-  HIPCHK(hipMemcpyAsync(L.st, &s0, sizeof s0, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(done, 0, sizeof(unsigned), st));
-  HIPCHK(hipMemsetAsync(pcount, 0, sizeof(long long) * kPhaseSlots, st));
-  HIPCHK(hipMemcpyAsync(d_lp, &L, sizeof L, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
-
-  h->precision = sizeof(T) == 8 ? 64 : 32;
-  h->W = W;
-  h->m = lp->m;
-  h->n = lp->n;
-  h->lpdev = d_lp;
-  h->st = L.st;
-  h->ctl = ctl;
-  h->done = done;
-  h->xo = L.xo; h->yo = L.yo; h->dout = L.dout; h->hist = L.hist; h->rhist = L.rhist;
-  h->pcount = pcount;
-  h->hist_cap = hist_cap;
-  h->ticks_set = c0.ticks;
-  // record the launches of one replay into a graph on a private capture
-  // stream: W == 1, one run launch (kPhasesPerLaunch phases); W > 1,
-  // kPairsPerGraph phase + decide pairs (one phase each)
-  HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
-  HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-  constexpr int TN = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
-  if (W == 1) {
-    pdlp_run_kernel<T, kBlock, TN><<<1, kBlock, 0, h->cap>>>(d_lp, ctl, done, kPhasesPerLaunch);
-    h->phases_per_replay = kPhasesPerLaunch;
-  } else {
-    for (int i = 0; i < kPairsPerGraph; ++i) {
-      pdlp_phase_kernel<T, kBlock, TN><<<W, kBlock, 0, h->cap>>>(d_lp);
-      pdlp_decide_kernel<T, kBlock><<<1, kBlock, 0, h->cap>>>(d_lp, W, ctl, done);
-    }
-    h->phases_per_replay = kPairsPerGraph;
-  }
-  HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
-  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
-  HIPCHK(hipHostMalloc((void**)&h->hstate, 2 * sizeof(LpState), hipHostMallocDefault));
-  for (hipEvent_t& e : h->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  return clock_hz(&h->hz);
-}
-
-template <class T>
-size_t batch_ws_T(int count, const sovlp_lp* lps, long long hist_cap) {
-  Carver cv{nullptr, 0, 0};
-  cv.take<LpDev<T>>(count);
-  cv.take<StepCtl>(1);
-  cv.take<unsigned>(64);
-  cv.take<long long>((size_t)count * kPhaseSlots);
-  for (int i = 0; i < count; ++i) {
-    LpDev<T> L{};
-    carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
-  }
-  return cv.off + 256;
-}
-
-template <class T>
-int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, const int32_t* blocks,
-                   void* ws, int64_t ws_bytes, int64_t hist_cap, hipStream_t st, sovlp_batch* h) {
-  if ((size_t)ws_bytes < batch_ws_T<T>(count, lps, hist_cap)) return fail(1, "workspace too small");
-  // descriptors grouped by workgroup size (one launch per group), largest LP
-  // first within a group, so that the longest workgroups of a launch start
-  // first and do not extend its tail; pos[i] is LP i's place in the device
-  // array (every output stays indexed by i)
-  const int kBlocks[3] = {64, 128, 256};
-  std::vector<int> order, pos(count), gcount(3, 0);
-  for (int g = 0; g < 3; ++g) {
-    std::vector<int> idx;
-    for (int i = 0; i < count; ++i)
-      if (blocks[i] == kBlocks[g]) idx.push_back(i);
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](int a, int b) { return lps[a].nnz > lps[b].nnz; });
-    for (int i : idx) {
-      pos[i] = (int)order.size();
-      order.push_back(i);
-      ++gcount[g];
-    }
-  }
-  if ((int)order.size() != count) return fail(1, "every LP's block must be 64, 128 or 256");
-  Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
-  LpDev<T>* d_lps = cv.take<LpDev<T>>(count);
-  StepCtl* ctl = cv.take<StepCtl>(1);
-  unsigned* done = cv.take<unsigned>(64);
-  long long* pcount = cv.take<long long>((size_t)count * kPhaseSlots);
-  std::vector<LpDev<T>> host(count);
-  h->xo.resize(count);
-  h->yo.resize(count);
-  h->dout.resize(count);
-  h->rhist.resize(count);
-  h->st.resize(count);
-  for (int i = 0; i < count; ++i) {
-    if (lps[i].nnz >= (int64_t)INT32_MAX) return fail(4, "an LP has more than 2^31-1 nonzeros");
-    LpDev<T>& L = host[pos[i]];
-    L = LpDev<T>{};
-    carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
-    fill_lp<T>(L, lps[i]);
-    L.pcount = pcount + (size_t)i * kPhaseSlots;
-    h->xo[i] = L.xo; h->yo[i] = L.yo; h->dout[i] = L.dout; h->st[i] = L.st;
-    h->rhist[i] = L.rhist;
-  }
-  StepCtl c0{};
-  std::memcpy(&c0.P, prm, sizeof(Params));
-  c0.ticks = (long long)9.0e18;
-  HIPCHK(hipMemcpyAsync(d_lps, host.data(), sizeof(LpDev<T>) * count, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
-  h->count = count;
-  h->precision = sizeof(T) == 8 ? 64 : 32;
-  h->block = blocks[0];
-  h->lps = d_lps;
-  h->ctl = ctl;
-  h->done = done;
-  h->pcount = pcount;
-  h->ticks_set = c0.ticks;
-  HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
-  // one launch per group; the groups are independent branches of the graph
-  // (forked from and joined to the capture stream), so they can run side by side
-  int first = -1, nside = 0;
-  for (int g = 0; g < 3; ++g)
-    if (gcount[g] > 0 && first < 0) first = g;
-  HIPCHK(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
-  for (int k = 0; k < 2; ++k) {
-    HIPCHK(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&h->join[k], hipEventDisableTiming));
-  }
-  HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-  HIPCHK(hipEventRecord(h->fork, h->cap));
-  for (int g = 0, off = 0; g < 3; off += gcount[g], ++g) {
-    if (gcount[g] == 0) continue;
-    if (g == first) {
-      launch_run<T>(kBlocks[g], d_lps + off, gcount[g], ctl, done, kPhasesPerLaunch, h->cap);
-    } else {
-      hipStream_t sd = h->side[nside];
-      HIPCHK(hipStreamWaitEvent(sd, h->fork, 0));
-      launch_run<T>(kBlocks[g], d_lps + off, gcount[g], ctl, done, kPhasesPerLaunch, sd);
-      HIPCHK(hipEventRecord(h->join[nside], sd));
-      ++nside;
-    }
-  }
-  for (int k = 0; k < nside; ++k) HIPCHK(hipStreamWaitEvent(h->cap, h->join[k], 0));
-  HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
-  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
-  HIPCHK(hipHostMalloc((void**)&h->hdone, 2 * sizeof(unsigned), hipHostMallocDefault));
-  for (hipEvent_t& e : h->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  return clock_hz(&h->hz);
-}
-
-int set_ticks(StepCtl* ctl, long long* cached, double seconds, double hz, hipStream_t st) {
-  const double secs = seconds > 0.0 ? seconds : 0.0;
-  long long ticks = (long long)std::min(secs * hz, 9.0e18);
-  if (ticks != *cached) {
-    *cached = ticks;
-    HIPCHK(hipMemcpyAsync(&ctl->ticks, cached, sizeof(long long), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));   // the source is host memory owned by the handle
-  }
-  return 0;
-}
-
-void fill_info(const LpState& t, sovlp_info* f) {
-  std::memset(f, 0, sizeof *f);
-  const bool opt = t.status == ST_OPTIMAL;
-  const double nan = std::nan("");
-  f->status = t.status;
-  f->stage = t.phase == PH_DONE ? 3
-           : t.phase < PH_KKT0 ? 0
-           : (t.phase == PH_FINAL || t.phase == PH_FINAL_D) ? 2 : 1;
-  f->iterations = t.k;
-  f->restarts = t.restarts;
-  f->rejected_steps = t.rejected;
-  f->history_len = t.hist_len;
-  f->primal_res = opt ? t.fin[0] : nan;
-  f->dual_res = opt ? t.fin[1] : nan;
-  f->gap = opt ? t.fin[2] : nan;
-  f->primal_obj = opt ? t.fin[3] : nan;
-  f->dual_obj = opt ? t.fin[4] : nan;
-  f->step_size = t.eta;
-  f->primal_weight = t.omega;
-  f->objective = t.objective;
-  f->scale_obj = t.sobj0;
-  f->device_ticks = t.ticks;
-  f->iterations = t.k_done + t.k;
-  f->refine_rows = t.ref_rows;
-  f->refine_round = t.round;
-  f->refine_scale = t.ref_scale;
-  f->start_tick = t.tick0;
-}
-
-int check_params(const sovlp_params* p, int precision) {
-  if (p->check_every < 1 || p->stepsize_every < 1) return fail(1, "check/stepsize period < 1");
-  if (p->refine_rounds < 0 || p->refine_rounds > kMaxRounds)
-    return fail(1, "refine_rounds must be in [0, 16]");
-  if (p->refine_rounds > 0 && precision != 32)
-    return fail(4, "iterative refinement is provided for precision 32 only");
-  if (p->refine_rounds > 0 && !(p->refine_eps > 0.0 && p->refine_tol >= 0.0))
-    return fail(1, "refine_eps must be > 0 and refine_tol >= 0");
-  if (p->refine_stall < 0 || p->refine_stall_open < 0)
-    return fail(1, "refine_stall and refine_stall_open must be >= 0");
-  return 0;
-}
-
-}  // namespace
-
-extern "C" {
-
-int sovlp_pdlp_workspace_bytes(const sovlp_lp* lp, int precision, int64_t history_cap,
-                               int64_t* bytes) {
-  if (!lp || !bytes) return fail(1, "null argument");
-  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  int W = 1;
-  int rc = precision == 64 ? pick_width<double>(lp, 2, &W) : pick_width<float>(lp, 2, &W);
-  if (rc) return rc;
-  *bytes = (int64_t)(precision == 64 ? single_ws_T<double>(lp, history_cap, W)
-                                     : single_ws_T<float>(lp, history_cap, W));
-  return 0;
-}
-
-int sovlp_pdlp_create(const sovlp_lp* lp, const sovlp_params* params, int precision,
-                      void* workspace, int64_t workspace_bytes, int64_t history_cap, int team,
-                      sovlp_stream_t stream, sovlp_pdlp** out) {
-  static_assert(sizeof(sovlp_params) == sizeof(Params), "params layout");
-  if (!lp || !params || !workspace || !out) return fail(1, "null argument");
-  if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) return fail(1, "negative dimension");
-  if (lp->nnz >= (int64_t)INT32_MAX) return fail(4, "more than 2^31-1 nonzeros");
-  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  if (int rc = check_params(params, precision)) return rc;
-  if (team < 0 || team > 2) return fail(1, "team must be 0 (auto), 1 (one workgroup) or 2 (many)");
-  sovlp_pdlp* h = new sovlp_pdlp{};
-  int rc = precision == 64
-               ? single_create_T<double>(lp, params, workspace, workspace_bytes, history_cap, team,
-                                         S(stream), h)
-               : single_create_T<float>(lp, params, workspace, workspace_bytes, history_cap, team,
-                                        S(stream), h);
-  if (rc) {
-    sovlp_pdlp_destroy(h);
-    return rc;
-  }
-  *out = h;
-  return 0;
-}
-
-int sovlp_pdlp_advance(sovlp_pdlp* h, int64_t iteration_budget, double seconds_left,
-                       sovlp_stream_t stream) {
-  if (!h) return fail(1, "null handle");
-  hipStream_t st = S(stream);
-  if (!h->started) {
-    int rc = set_ticks(h->ctl, &h->ticks_set, seconds_left, h->hz, st);
-    if (rc) return rc;
-    h->started = true;
-  }
-  HIPCHK(hipMemcpyAsync(&h->hstate[0], h->st, sizeof(LpState), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  const long long k0 = h->hstate[0].k;
-  const long long budget = iteration_budget < 1 ? 1 : iteration_budget;
-  // every replay runs phases_per_replay phases; an iteration takes at most 61
-  // phases (60 rejected trials), so this bound is never the one that stops a
-  // healthy solve
-  const long long max_replays = (budget * 61 + 200) / h->phases_per_replay + 16;
-  // one replay stays queued while the host reads the state the previous one
-  // left (pinned copy + event), so the device does not idle on the host
-  long long replays = 0;
-  int cur = 0;
-  if (h->hstate[0].phase == PH_DONE) return 0;
-  HIPCHK(hipGraphLaunch(h->exec, st));
-  HIPCHK(hipMemcpyAsync(&h->hstate[0], h->st, sizeof(LpState), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipEventRecord(h->ev[0], st));
-  ++replays;
-  for (;;) {
-    const int nxt = cur ^ 1;
-    const bool more = replays < max_replays;
-    if (more) {
-      HIPCHK(hipGraphLaunch(h->exec, st));
-      HIPCHK(hipMemcpyAsync(&h->hstate[nxt], h->st, sizeof(LpState), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipEventRecord(h->ev[nxt], st));
-      ++replays;
-    }
-    HIPCHK(hipEventSynchronize(h->ev[cur]));
-    const LpState& s = h->hstate[cur];
-    if (s.phase == PH_DONE || s.k - k0 >= budget || !more) break;
-    cur = nxt;
-  }
-  HIPCHK(hipStreamSynchronize(st));
-  return 0;
-}
-
-int sovlp_pdlp_info(sovlp_pdlp* h, sovlp_info* info, sovlp_stream_t stream) {
-  if (!h || !info) return fail(1, "null argument");
-  LpState s;
-  HIPCHK(hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, S(stream)));
-  HIPCHK(hipStreamSynchronize(S(stream)));
-  fill_info(s, info);
-  return 0;
-}
-
-int sovlp_pdlp_outputs(sovlp_pdlp* h, double** x, double** y, double** d, double** history) {
-  if (!h) return fail(1, "null handle");
-  if (x) *x = h->xo;
-  if (y) *y = h->yo;
-  if (d) *d = h->dout;
-  if (history) *history = h->hist;
-  return 0;
-}
-
-int sovlp_pdlp_refinement(sovlp_pdlp* h, double** rows) {
-  if (!h || !rows) return fail(1, "null argument");
-  *rows = h->rhist;
-  return 0;
-}
-
-int sovlp_pdlp_width(sovlp_pdlp* h, int* workgroups) {
-  if (!h || !workgroups) return fail(1, "null argument");
-  *workgroups = h->W;
-  return 0;
-}
-
-int sovlp_phase_slots(int* slots) {
-  if (!slots) return fail(1, "null argument");
-  *slots = kPhaseSlots;
-  return 0;
-}
-
-const char* sovlp_phase_name(int slot) {
-  static const char* names[kPhaseSlots] = {
-      "INIT", "RUIZ_MAX", "RUIZ_APPLY", "PC_SUM", "PC_APPLY", "SCALE", "AX0", "POW_A",
-      "POW_T", "KKT0", "P_TRIAL", "A_TRIAL", "B_ADAPT", "A_FUSED", "B_FUSED", "CHECK",
-      "RESTART", "COPY_AVG", "FINAL", "FINAL_D", "REF_X", "REF_KKT", "REF_SHIFT", "DONE",
-      "B_ADAPT_NEXT", "B_FUSED_NEXT", "", ""};
-  return slot >= 0 && slot < kPhaseSlots ? names[slot] : "";
-}
-
-int sovlp_pdlp_phase_counts(sovlp_pdlp* h, int64_t* counts, sovlp_stream_t stream) {
-  if (!h || !counts) return fail(1, "null argument");
-  static_assert(sizeof(long long) == sizeof(int64_t), "counter width");
-  HIPCHK(hipMemcpyAsync(counts, h->pcount, sizeof(long long) * kPhaseSlots,
-                        hipMemcpyDeviceToHost, S(stream)));
-  HIPCHK(hipStreamSynchronize(S(stream)));
-  return 0;
-}
-
-int sovlp_pdlp_destroy(sovlp_pdlp* h) {
-  if (!h) return 0;
-  for (hipEvent_t e : h->ev)
-    if (e) (void)hipEventDestroy(e);
-  if (h->hstate) (void)hipHostFree(h->hstate);
-  if (h->exec) (void)hipGraphExecDestroy(h->exec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  if (h->cap) (void)hipStreamDestroy(h->cap);
-  delete h;
-  return 0;
-}
-
-int sovlp_batch_geometry(int precision, int block, int* tile_rows, int* tile_nnz) {
-  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
-  *tile_rows = kRowsPerLane * block;
-  *tile_nnz = batch_tnnz(precision, block);
-  return 0;
-}
-
-int sovlp_batch_workspace_bytes(int count, const sovlp_lp* lps, int precision,
-                                int64_t history_cap, int64_t* bytes) {
-  if (count < 1 || !lps || !bytes) return fail(1, "bad batch arguments");
-  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  *bytes = (int64_t)(precision == 64 ? batch_ws_T<double>(count, lps, history_cap)
-                                     : batch_ws_T<float>(count, lps, history_cap));
-  return 0;
-}
-
-int sovlp_batch_create_mixed(int count, const sovlp_lp* lps, const sovlp_params* params,
-                             int precision, const int32_t* blocks, void* workspace,
-                             int64_t workspace_bytes, int64_t history_cap, sovlp_stream_t stream,
-                             sovlp_batch** out) {
-  if (count < 1 || !lps || !params || !blocks || !workspace || !out)
-    return fail(1, "bad batch arguments");
-  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  for (int i = 0; i < count; ++i)
-    if (!batch_block_ok(blocks[i])) return fail(1, "block must be 64, 128 or 256");
-  if (int rc = check_params(params, precision)) return rc;
-  sovlp_batch* h = new sovlp_batch{};
-  int rc = precision == 64
-               ? batch_create_T<double>(count, lps, params, blocks, workspace, workspace_bytes,
-                                        history_cap, S(stream), h)
-               : batch_create_T<float>(count, lps, params, blocks, workspace, workspace_bytes,
-                                       history_cap, S(stream), h);
-  if (rc) {
-    sovlp_batch_destroy(h);
-    return rc;
-  }
-  *out = h;
-  return 0;
-}
-
-int sovlp_batch_create(int count, const sovlp_lp* lps, const sovlp_params* params,
-                       int precision, int block, void* workspace, int64_t workspace_bytes,
-                       int64_t history_cap, sovlp_stream_t stream, sovlp_batch** out) {
-  if (count < 1) return fail(1, "bad batch arguments");
-  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
-  std::vector<int32_t> blocks(count, block);
-  return sovlp_batch_create_mixed(count, lps, params, precision, blocks.data(), workspace,
-                                  workspace_bytes, history_cap, stream, out);
-}
-
-int sovlp_batch_run(sovlp_batch* h, double seconds_per_lp, int64_t max_graphs,
-                    sovlp_stream_t stream, int64_t* graphs_run) {
-  if (!h) return fail(1, "null handle");
-  hipStream_t st = S(stream);
-  int rc = set_ticks(h->ctl, &h->ticks_set, seconds_per_lp, h->hz, st);
-  if (rc) return rc;
-  if (h->precision == 64)
-    state_reset_kernel<double><<<(h->count + 255) / 256, 256, 0, st>>>(
-        (const LpDev<double>*)h->lps, h->count, h->done);
-  else
-    state_reset_kernel<float><<<(h->count + 255) / 256, 256, 0, st>>>(
-        (const LpDev<float>*)h->lps, h->count, h->done);
-  HIPCHK(hipGetLastError());
-  // one replay stays queued while the host reads the done counter the
-  // previous one left (pinned copy + event)
-  long long g = 0;
-  int cur = 0;
-  HIPCHK(hipGraphLaunch(h->exec, st));
-  HIPCHK(hipMemcpyAsync(&h->hdone[0], h->done, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipEventRecord(h->ev[0], st));
-  ++g;
-  for (;;) {
-    const int nxt = cur ^ 1;
-    const bool more = max_graphs <= 0 || g < max_graphs;
-    if (more) {
-      HIPCHK(hipGraphLaunch(h->exec, st));
-      HIPCHK(hipMemcpyAsync(&h->hdone[nxt], h->done, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipEventRecord(h->ev[nxt], st));
-      ++g;
-    }
-    HIPCHK(hipEventSynchronize(h->ev[cur]));
-    if (h->hdone[cur] >= (unsigned)h->count || !more) break;
-    cur = nxt;
-  }
-  HIPCHK(hipStreamSynchronize(st));
-  if (graphs_run) *graphs_run = g;
-  return 0;
-}
-
-int sovlp_batch_infos(sovlp_batch* h, sovlp_info* infos, sovlp_stream_t stream) {
-  if (!h || !infos) return fail(1, "null argument");
-  std::vector<LpState> s(h->count);
-  for (int i = 0; i < h->count; ++i)
-    HIPCHK(hipMemcpyAsync(&s[i], h->st[i], sizeof(LpState), hipMemcpyDeviceToHost, S(stream)));
-  HIPCHK(hipStreamSynchronize(S(stream)));
-  for (int i = 0; i < h->count; ++i) fill_info(s[i], &infos[i]);
-  return 0;
-}
-
-int sovlp_batch_outputs(sovlp_batch* h, int index, double** x, double** y, double** d) {
-  if (!h || index < 0 || index >= h->count) return fail(1, "bad batch index");
-  if (x) *x = h->xo[index];
-  if (y) *y = h->yo[index];
-  if (d) *d = h->dout[index];
-  return 0;
-}
-
-int sovlp_batch_refinement(sovlp_batch* h, int index, double** rows) {
-  if (!h || !rows || index < 0 || index >= h->count) return fail(1, "bad batch index");
-  *rows = h->rhist[index];
-  return 0;
-}
-
-int sovlp_batch_phase_counts(sovlp_batch* h, int64_t* counts, sovlp_stream_t stream) {
-  if (!h || !counts) return fail(1, "null argument");
-  HIPCHK(hipMemcpyAsync(counts, h->pcount, sizeof(long long) * kPhaseSlots * (size_t)h->count,
-                        hipMemcpyDeviceToHost, S(stream)));
-  HIPCHK(hipStreamSynchronize(S(stream)));
-  return 0;
-}
-
-int sovlp_batch_destroy(sovlp_batch* h) {
-  if (!h) return 0;
-  for (int k = 0; k < 2; ++k) {
-    if (h->side[k]) (void)hipStreamDestroy(h->side[k]);
-    if (h->join[k]) (void)hipEventDestroy(h->join[k]);
-  }
-  if (h->fork) (void)hipEventDestroy(h->fork);
-  for (hipEvent_t e : h->ev)
-    if (e) (void)hipEventDestroy(e);
-  if (h->hdone) (void)hipHostFree(h->hdone);
-  if (h->exec) (void)hipGraphExecDestroy(h->exec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  if (h->cap) (void)hipStreamDestroy(h->cap);
-  delete h;
-  return 0;
-}
-
-}  // extern "C"
-
-// ====================================================================== server
-// A slot table for continuous batching: `slots` LPs share every launch of one
-// recorded graph (one workgroup per slot, up to `phases` phases each).  An
-// LP is admitted into a free slot between launches (its descriptor and its
-// fresh state are copied in stream order), runs until its state reaches
-// DONE, and is released by the host after the result is read.  Empty and
-// finished slots point at a DONE state in the server's own workspace, so
-// their workgroups return at once and never touch a released LP's memory.
-struct sovlp_server {
-  int slots, precision, block, phases;
-  int team;                  // workgroups per slot and phase (1: run kernel)
-  double* partials;          // [slots][team][kMaxRed] (team > 1)
-  int32_t* parts;            // [slots][2][team + 1] tile partitions of A and A^T (team > 1)
-  int32_t* hparts;           // pinned staging of the partitions
-  void* lps;                 // LpDev<T>[slots] (device)
-  LpState* states;           // [slots] (device)
-  long long* pcount;         // [slots][kPhaseSlots] (device)
-  StepCtl* ctl;
-  unsigned* done;
-  char* hstage;              // pinned LpDev<T>[slots]: descriptors being copied in
-  LpState* hinit;            // pinned [slots]: fresh states being copied in
-  LpState* hsnap;            // pinned [slots]: states after the last step
-  std::vector<char> busy, seen_done;
-  std::vector<double*> xo, yo, dout, rhist;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-  hipStream_t cap;
-  double hz;
-};
-
-namespace {
-
-template <class T>
-size_t server_ws_T(int slots, int team) {
-  Carver cv{nullptr, 0, 0};
-  cv.take<LpDev<T>>(slots);
-  cv.take<LpState>(slots);
-  cv.take<long long>((size_t)slots * kPhaseSlots);
-  cv.take<StepCtl>(1);
-  cv.take<unsigned>(64);
-  if (team > 1) {
-    cv.take<double>((size_t)slots * team * kMaxRed);
-    cv.take<int32_t>((size_t)slots * 2 * (team + 1));
-  }
-  return cv.off + 256;
-}
-
-template <class T>
-size_t server_lp_ws_T(const sovlp_lp* lp, long long hist_cap) {
-  Carver cv{nullptr, 0, 0};
-  LpDev<T> L{};
-  carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
-  return cv.off + 256;
-}
-
-template <class T>
-int server_create_T(int slots, int team, const sovlp_params* prm, int block, int phases, void* ws,
-                    int64_t ws_bytes, hipStream_t st, sovlp_server* h) {
-  if ((size_t)ws_bytes < server_ws_T<T>(slots, team)) return fail(1, "workspace too small");
-  Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
-  LpDev<T>* d_lps = cv.take<LpDev<T>>(slots);
-  LpState* states = cv.take<LpState>(slots);
-  long long* pcount = cv.take<long long>((size_t)slots * kPhaseSlots);
-  StepCtl* ctl = cv.take<StepCtl>(1);
-  unsigned* done = cv.take<unsigned>(64);
-  h->team = team;
-  if (team > 1) {
-    h->partials = cv.take<double>((size_t)slots * team * kMaxRed);
-    h->parts = cv.take<int32_t>((size_t)slots * 2 * (team + 1));
-    HIPCHK(hipHostMalloc((void**)&h->hparts, sizeof(int32_t) * (size_t)slots * 2 * (team + 1),
-                         hipHostMallocDefault));
-  }
-  h->slots = slots;
-  h->precision = sizeof(T) == 8 ? 64 : 32;
-  h->block = block;
-  h->phases = phases;
-  h->lps = d_lps;
-  h->states = states;
-  h->pcount = pcount;
-  h->ctl = ctl;
-  h->done = done;
-  h->busy.assign(slots, 0);
-  h->seen_done.assign(slots, 0);
-  h->xo.assign(slots, nullptr);
-  h->yo.assign(slots, nullptr);
-  h->dout.assign(slots, nullptr);
-  h->rhist.assign(slots, nullptr);
-  HIPCHK(hipHostMalloc((void**)&h->hstage, sizeof(LpDev<T>) * (size_t)slots, hipHostMallocDefault));
-  HIPCHK(hipHostMalloc((void**)&h->hinit, sizeof(LpState) * (size_t)slots, hipHostMallocDefault));
-  HIPCHK(hipHostMalloc((void**)&h->hsnap, sizeof(LpState) * (size_t)slots, hipHostMallocDefault));
-  LpDev<T>* stage = reinterpret_cast<LpDev<T>*>(h->hstage);
-  for (int i = 0; i < slots; ++i) {
-    stage[i] = LpDev<T>{};
-    stage[i].st = states + i;
-    stage[i].pcount = pcount + (size_t)i * kPhaseSlots;
-    h->hinit[i] = LpState{};
-    h->hinit[i].phase = PH_DONE;
-    h->hinit[i].status = ST_RUNNING;
-  }
-  StepCtl c0{};
-  std::memcpy(&c0.P, prm, sizeof(Params));
-  const double secs = prm->time_limit > 0.0 ? prm->time_limit : 0.0;
-  double hz = 0.0;
-  if (int rc = clock_hz(&hz)) return rc;
-  h->hz = hz;
-  c0.ticks = (long long)std::min(secs * hz, 9.0e18);
-  HIPCHK(hipMemcpyAsync(d_lps, stage, sizeof(LpDev<T>) * (size_t)slots, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(states, h->hinit, sizeof(LpState) * (size_t)slots, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(pcount, 0, sizeof(long long) * kPhaseSlots * (size_t)slots, st));
-  HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(done, 0, sizeof(unsigned), st));
-  HIPCHK(hipStreamSynchronize(st));
-  HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
-  HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-  if (team > 1) {
-    // `phases` pairs: one phase of every slot over `team` workgroups, then one
-    // workgroup per slot reduces the partial sums and takes the transition
-    constexpr int TN = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
-    for (int i = 0; i < phases; ++i) {
-      pdlp_slot_phase_kernel<T, kBlock, TN><<<slots * team, kBlock, 0, h->cap>>>(d_lps, team);
-      pdlp_slot_decide_kernel<T, kBlock><<<slots, kBlock, 0, h->cap>>>(d_lps, team, ctl, done);
-    }
-  } else {
-    launch_run<T>(block, d_lps, slots, ctl, done, phases, h->cap);
-  }
-  HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
-  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
-  return 0;
-}
-
-template <class T>
-int server_admit_T(sovlp_server* h, int slot, const sovlp_lp* lp, const double* x0,
-                   const double* y0, double omega0, void* ws, int64_t ws_bytes,
-                   int64_t hist_cap, hipStream_t st) {
-  if ((size_t)ws_bytes < server_lp_ws_T<T>(lp, hist_cap)) return fail(1, "LP workspace too small");
-  Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
-  LpDev<T> L{};
-  carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
-  fill_lp<T>(L, *lp);
-  L.st = h->states + slot;   // the state lives in the server's table, not the LP's workspace
-  L.pcount = h->pcount + (size_t)slot * kPhaseSlots;
-  if (h->team > 1) {
-    // equal-cost split of both tile lists over the team (host plan from the
-    // device copies of the tile starts and row pointers)
-    const int W = h->team;
-    std::vector<int32_t> ta(lp->ntiles_a + 1), tt(lp->ntiles_t + 1), rp(lp->m + 1), cp(lp->n + 1);
-    HIPCHK(hipMemcpyAsync(ta.data(), lp->tiles_a, sizeof(int32_t) * ta.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(tt.data(), lp->tiles_t, sizeof(int32_t) * tt.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(rp.data(), lp->rp, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(cp.data(), lp->cp, sizeof(int32_t) * cp.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<int64_t> pa(ta.size()), pt(tt.size());
-    for (size_t t = 0; t < ta.size(); ++t) pa[t] = rp[ta[t]];
-    for (size_t t = 0; t < tt.size(); ++t) pt[t] = cp[tt[t]];
-    int32_t* hp = h->hparts + (size_t)slot * 2 * (W + 1);
-    host_partition(lp->ntiles_a, ta.data(), pa.data(), W, hp);
-    host_partition(lp->ntiles_t, tt.data(), pt.data(), W, hp + W + 1);
-    int32_t* dp = h->parts + (size_t)slot * 2 * (W + 1);
-    HIPCHK(hipMemcpyAsync(dp, hp, sizeof(int32_t) * 2 * (W + 1), hipMemcpyHostToDevice, st));
-    L.partA = dp;
-    L.partT = dp + W + 1;
-    L.partials = h->partials + (size_t)slot * W * kMaxRed;
-  }
-  L.wx0 = x0;
-  L.wy0 = y0;
-  L.womega = omega0 > 0.0 ? omega0 : 0.0;
-  LpDev<T>* stage = reinterpret_cast<LpDev<T>*>(h->hstage);
-  stage[slot] = L;
-  h->hinit[slot] = LpState{};
-  h->hinit[slot].phase = PH_INIT;
-  h->hinit[slot].status = ST_RUNNING;
-  LpDev<T>* d_lps = reinterpret_cast<LpDev<T>*>(h->lps);
-  HIPCHK(hipMemcpyAsync(d_lps + slot, &stage[slot], sizeof(LpDev<T>), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(h->states + slot, &h->hinit[slot], sizeof(LpState), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(h->pcount + (size_t)slot * kPhaseSlots, 0, sizeof(long long) * kPhaseSlots, st));
-  h->xo[slot] = L.xo;
-  h->yo[slot] = L.yo;
-  h->dout[slot] = L.dout;
-  h->rhist[slot] = L.rhist;
-  return 0;
-}
-
-}  // namespace
-
-extern "C" {
-
-int sovlp_server_workspace_bytes(int slots, int precision, int64_t* bytes) {
-  return sovlp_server_team_workspace_bytes(slots, 1, precision, bytes);
-}
-
-int sovlp_server_team_workspace_bytes(int slots, int team, int precision, int64_t* bytes) {
-  if (slots < 1 || team < 1 || !bytes) return fail(1, "slots and team must be >= 1");
-  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  *bytes = (int64_t)(precision == 64 ? server_ws_T<double>(slots, team)
-                                     : server_ws_T<float>(slots, team));
-  return 0;
-}
-
-int sovlp_server_lp_bytes(const sovlp_lp* lp, int precision, int64_t history_cap,
-                          int64_t* bytes) {
-  if (!lp || !bytes) return fail(1, "null argument");
-  if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) return fail(1, "negative dimension");
-  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  *bytes = (int64_t)(precision == 64 ? server_lp_ws_T<double>(lp, history_cap)
-                                     : server_lp_ws_T<float>(lp, history_cap));
-  return 0;
-}
-
-static int server_create(int slots, int team, const sovlp_params* params, int precision,
-                         int block, int phases_per_launch, void* workspace,
-                         int64_t workspace_bytes, sovlp_stream_t stream, sovlp_server** out) {
-  if (slots < 1 || team < 1 || !params || !workspace || !out)
-    return fail(1, "bad server arguments");
-  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
-  if (phases_per_launch < 1 || phases_per_launch > 4096)
-    return fail(1, "phases_per_launch must be in [1, 4096]");
-  if ((long long)slots * team > (1LL << 24)) return fail(1, "slots x team too large");
-  if (int rc = check_params(params, precision)) return rc;
-  sovlp_server* h = new sovlp_server{};
-  int rc = precision == 64
-               ? server_create_T<double>(slots, team, params, block, phases_per_launch, workspace,
-                                         workspace_bytes, S(stream), h)
-               : server_create_T<float>(slots, team, params, block, phases_per_launch, workspace,
-                                        workspace_bytes, S(stream), h);
-  if (rc) {
-    sovlp_server_destroy(h);
-    return rc;
-  }
-  *out = h;
-  return 0;
-}
-
-int sovlp_server_create(int slots, const sovlp_params* params, int precision, int block,
-                        int phases_per_launch, void* workspace, int64_t workspace_bytes,
-                        sovlp_stream_t stream, sovlp_server** out) {
-  return server_create(slots, 1, params, precision, block, phases_per_launch, workspace,
-                       workspace_bytes, stream, out);
-}
-
-int sovlp_server_create_team(int slots, int team, const sovlp_params* params, int precision,
-                             int pairs_per_launch, void* workspace, int64_t workspace_bytes,
-                             sovlp_stream_t stream, sovlp_server** out) {
-  return server_create(slots, team, params, precision, kBlock, pairs_per_launch, workspace,
-                       workspace_bytes, stream, out);
-}
-
-int sovlp_server_admit(sovlp_server* h, int slot, const sovlp_lp* lp, const double* x0,
-                       const double* y0, double omega0, void* workspace,
-                       int64_t workspace_bytes, int64_t history_cap, sovlp_stream_t stream) {
-  if (!h || !lp || !workspace) return fail(1, "null argument");
-  if (slot < 0 || slot >= h->slots) return fail(1, "slot out of range");
-  if (h->busy[slot]) return fail(1, "slot is occupied (release it first)");
-  if (lp->m < 0 || lp->n < 0 || lp->nnz < 0) return fail(1, "negative dimension");
-  if (lp->nnz >= (int64_t)INT32_MAX) return fail(4, "more than 2^31-1 nonzeros");
-  int rc = h->precision == 64
-               ? server_admit_T<double>(h, slot, lp, x0, y0, omega0, workspace, workspace_bytes,
-                                        history_cap, S(stream))
-               : server_admit_T<float>(h, slot, lp, x0, y0, omega0, workspace, workspace_bytes,
-                                       history_cap, S(stream));
-  if (rc) return rc;
-  h->busy[slot] = 1;
-  h->seen_done[slot] = 0;
-  return 0;
-}
-
-int sovlp_server_step(sovlp_server* h, int launches, sovlp_stream_t stream, int32_t* finished,
-                      int32_t* running) {
-  if (!h) return fail(1, "null handle");
-  hipStream_t st = S(stream);
-  for (int i = 0; i < launches; ++i) HIPCHK(hipGraphLaunch(h->exec, st));
-  HIPCHK(hipMemcpyAsync(h->hsnap, h->states, sizeof(LpState) * (size_t)h->slots,
-                        hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  int run = 0;
-  for (int i = 0; i < h->slots; ++i) {
-    const bool fin = h->busy[i] && h->hsnap[i].phase == PH_DONE;
-    if (fin) h->seen_done[i] = 1;
-    if (finished) finished[i] = fin ? 1 : 0;
-    run += (h->busy[i] && !fin) ? 1 : 0;
-  }
-  if (running) *running = run;
-  return 0;
-}
-
-int sovlp_server_info(sovlp_server* h, int slot, sovlp_info* info) {
-  if (!h || !info) return fail(1, "null argument");
-  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
-  fill_info(h->hsnap[slot], info);
-  return 0;
-}
-
-int sovlp_server_outputs(sovlp_server* h, int slot, double** x, double** y, double** d) {
-  if (!h) return fail(1, "null handle");
-  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
-  if (x) *x = h->xo[slot];
-  if (y) *y = h->yo[slot];
-  if (d) *d = h->dout[slot];
-  return 0;
-}
-
-int sovlp_server_refinement(sovlp_server* h, int slot, double** rows) {
-  if (!h || !rows) return fail(1, "null argument");
-  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
-  *rows = h->rhist[slot];
-  return 0;
-}
-
-int sovlp_server_phase_counts(sovlp_server* h, int slot, int64_t* counts,
-                              sovlp_stream_t stream) {
-  if (!h || !counts) return fail(1, "null argument");
-  if (slot < 0 || slot >= h->slots) return fail(1, "slot out of range");
-  HIPCHK(hipMemcpyAsync(counts, h->pcount + (size_t)slot * kPhaseSlots,
-                        sizeof(long long) * kPhaseSlots, hipMemcpyDeviceToHost, S(stream)));
-  HIPCHK(hipStreamSynchronize(S(stream)));
-  return 0;
-}
-
-int sovlp_server_release(sovlp_server* h, int slot) {
-  if (!h) return fail(1, "null handle");
-  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
-  if (!h->seen_done[slot]) return fail(1, "the slot's LP has not finished (step until it has)");
-  h->busy[slot] = 0;
-  h->seen_done[slot] = 0;
-  h->xo[slot] = h->yo[slot] = h->dout[slot] = h->rhist[slot] = nullptr;
-  return 0;
-}
-
-int sovlp_server_clock_hz(sovlp_server* h, double* hz) {
-  if (!h || !hz) return fail(1, "null argument");
-  *hz = h->hz;
-  return 0;
-}
-
-int sovlp_server_destroy(sovlp_server* h) {
-  if (!h) return 0;
-  if (h->exec) (void)hipGraphExecDestroy(h->exec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  if (h->cap) (void)hipStreamDestroy(h->cap);
-  if (h->hstage) (void)hipHostFree(h->hstage);
-  if (h->hinit) (void)hipHostFree(h->hinit);
-  if (h->hsnap) (void)hipHostFree(h->hsnap);
-  if (h->hparts) (void)hipHostFree(h->hparts);
-  delete h;
-  return 0;
-}
-
-}  // extern "C"

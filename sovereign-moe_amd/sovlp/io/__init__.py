@@ -1,21 +1,17 @@
-"""Model files (reference ``src/sovopt/io/__init__.py:12-25``): MPS in and out,
-CPLEX LP in.  ``read_model`` picks the reader by extension, looking through
-compression suffixes: ``.lp`` to ``read_lp``, everything else to ``read_mps``.
+"""Model files: MPS in and out, CPLEX LP in.  ``read_model`` is the entry the
+reference offers at ``src/sovopt/io/__init__.py:12-25``: the reader follows the
+format extension, found through the compression table of ``io/mps.py``.
 """
 
 from .lp_format import LPFormatError, read_lp
-from .mps import MPSError, read_mps, write_mps
+from .mps import MPSError, _format_extension, read_mps, write_mps
 
 __all__ = ["read_model", "read_mps", "write_mps", "read_lp", "MPSError", "LPFormatError"]
 
+_READERS = {".lp": read_lp}
+
 
 def read_model(path, name=None):
-    """Read a model file (``.mps`` or ``.lp``, optionally ``.gz``, ``.bz2``, ``.xz``)."""
-    base = str(path).lower()
-    for suf in (".gz", ".bz2", ".xz", ".lzma"):
-        if base.endswith(suf):
-            base = base[: -len(suf)]
-            break
-    if base.endswith(".lp"):
-        return read_lp(path, name=name)
-    return read_mps(path, name=name)
+    """Read a model file: CPLEX LP for ``.lp``, MPS otherwise; either may be
+    compressed (``.gz``, ``.bz2``, ``.xz``, ``.lzma``)."""
+    return _READERS.get(_format_extension(path), read_mps)(path, name=name)

@@ -66,6 +66,15 @@ def _text(path):
     return open(path, "r", encoding="utf-8", errors="replace")
 
 
+def _format_extension(path) -> str:
+    """Lower-case extension of the model held in ``path``, looked up through one
+    compression suffix of ``_COMPRESSED`` (``plan.lp.gz`` -> ``.lp``)."""
+    stem, ext = os.path.splitext(os.path.basename(str(path)))
+    if ext.lower() in _COMPRESSED:
+        ext = os.path.splitext(stem)[1]
+    return ext.lower()
+
+
 def _model_name(path) -> str:
     base = os.path.basename(str(path))
     stem, ext = os.path.splitext(base)
