@@ -24,17 +24,31 @@ Beside the headline number the JSON line carries:
                 the launch stream), against 8 TB/s; `traffic` the HBM bytes per
                 launch from the committed FETCH_SIZE / WRITE_SIZE passes when
                 they were taken on this workload and launch structure
-  cpu_baseline  the FP64 C restatement of the reference solver on a bounded
-                sample of the batch, one LP per host thread
-  spmv          the tile-pass SpMV of A and A^T of the 200k x 1M skewed LP, in
-                the given and the locality column order (optional, --spmv)
+  cpu_baseline  the FP64 C restatement of the reference solver: the batch
+                sample one LP per host thread (the headline's unit), the same
+                on one thread, and the 500 x 2000 LP's wall time per solve
+                (median of 3 after a warm-up) on the affinity's cores and on one
+  spmv          the tile-pass SpMV of A and A^T of the 200k x 1M skewed LP,
+                FP64 and FP32, in the given and the locality column order
+                (HIP events, algorithmic GB/s against 8 TB/s)
+  large_lp      the 200k x 1M skewed LP solved once in FP64 (rel KKT 1e-6)
+                and once in FP32 + iterative refinement: status, iterations,
+                device time, objective error against the closed form, and the
+                refinement record
   bnr           batched node relaxation, 128 nodes of a 500 x 2000 LP
                 (optional, --bnr)
   certify       the certificate of a point of the 200k x 1M LP (optional,
                 --certify)
+  serve         a Poisson stream of multi-period blending LPs through the
+                serving loop (optional, --serve)
+
+`--only SECTION` runs one section by itself (spmv, large_lp, bnr, certify,
+serve, or copy: a plain 256 MiB device copy whose counter readings calibrate
+the FETCH_SIZE / WRITE_SIZE passes) and prints its JSON: the short runs the
+committed rocprofv3 profiles were taken on.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lps 4096]
-                       [--precision 64|32] [--block 256|64]
+                       [--precision 64|32] [--block 256|64] [--only SECTION]
 """
 
 from __future__ import annotations
@@ -81,7 +95,26 @@ def parse():
                     help="LPs solved one at a time for the FP64 single-LP rate (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="wall budget of the CPU baseline sample (0: skip)")
-    ap.add_argument("--spmv", action="store_true", help="also time the SpMV of the 200k x 1M LP")
+    ap.add_argument("--no-spmv", action="store_true",
+                    help="skip the SpMV section (A and A^T of the 200k x 1M LP)")
+    ap.add_argument("--no-large-lp", action="store_true",
+                    help="skip the 200k x 1M LP solves (FP64, FP32 + refinement)")
+    ap.add_argument("--panels", action="store_true",
+                    help="SpMV section: also A x as eight XCD column panels")
+    ap.add_argument("--large-lp-locality", action="store_true",
+                    help="large LP: also solve with the columns in locality order")
+    ap.add_argument("--serve", action="store_true",
+                    help="also run a Poisson stream of blending LPs through the serving loop")
+    ap.add_argument("--serve-rate", type=float, default=10.0)
+    ap.add_argument("--serve-duration", type=float, default=10.0)
+    ap.add_argument("--serve-tmin", type=int, default=24)
+    ap.add_argument("--serve-tmax", type=int, default=36)
+    ap.add_argument("--serve-slots", type=int, default=128)
+    ap.add_argument("--serve-team", type=int, default=8)
+    ap.add_argument("--serve-precision", type=int, default=64, choices=(32, 64))
+    ap.add_argument("--only", default="",
+                    choices=("", "spmv", "large_lp", "bnr", "certify", "serve", "copy"),
+                    help="run only this section and print its JSON")
     ap.add_argument("--bnr", action="store_true",
                     help="also time a batched node relaxation: 128 nodes of the 500 x 2000 LP, "
                          "60 PDHG iterations, and the oracle on a sample")
@@ -181,7 +214,7 @@ def bnr_rate(K: int = 128, iters: int = 60, reps: int = 10) -> dict:
     return out
 
 
-def certify_rate(reps: int = 20) -> dict:
+def certify_rate(reps: int = 20, k=None) -> dict:
     """One certificate of a point (x inside the box, y random) of the 200k x 1M
     skewed LP: device time per call (kernels, the 48-byte result copy and the
     synchronisation), its algorithmic bytes, and the C oracle on the same call
@@ -191,7 +224,7 @@ def certify_rate(reps: int = 20) -> dict:
     from sovlp.lp.certify import DeviceCertifier
     from sovlp.workloads import skewed_lp
     t0 = time.perf_counter()
-    k = skewed_lp(seed=1)
+    k = skewed_lp(seed=1) if k is None else k
     p = k.problem
     rng = np.random.default_rng(9)
     x = rng.uniform(0.0, 1.0, p.n) * np.minimum(p.col_ub, 10.0)
@@ -278,33 +311,191 @@ def plain_fp64_rate(problems, tol, sample):
 
 
 def cpu_baseline(problems, tol, budget_s):
-    """The FP64 C restatement of the reference solver, one LP per host thread,
-    over as many LPs of the batch as finish within about ``budget_s``."""
+    """The FP64 C restatement of the reference solver (the oracle).
+
+    * ``value``: LPs/s over the batch, one LP per host thread, as many LPs as
+      finish within about ``budget_s``, on the affinity's cores (at most 16,
+      the host share of one GPU here; ``cores`` says how many ran);
+    * ``one_thread``: the same on one thread, a quarter of the budget;
+    * ``configs0``: the 500 x 2000 LP at rel KKT 1e-8, wall time per solve,
+      median of 3 after one warm-up, with its OpenMP products on the affinity's
+      cores and on one thread."""
     from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
     from oracle import oracle
+    from sovlp.workloads import small_lp
     oracle.lib()
-    cores = len(os.sched_getaffinity(0))
-    cores = max(1, min(cores, 16))
-    done = []
-    t0 = time.perf_counter()
-    deadline = t0 + budget_s
+    affinity = len(os.sched_getaffinity(0))
+    cap = 16
+    cores = max(1, min(affinity, cap))
 
-    def work(i):
-        if time.perf_counter() > deadline:
-            return None
-        r = oracle.solve(problems[i], eps_abs=tol, eps_rel=tol, feas_cap=tol, threads=1,
-                         hist_cap=1)
-        return r.status
+    def rate(threads, budget):
+        done = []
+        t0 = time.perf_counter()
+        deadline = t0 + budget
 
-    with ThreadPoolExecutor(cores) as ex:
-        for st in ex.map(work, range(len(problems))):
-            if st is not None:
-                done.append(st)
-    el = time.perf_counter() - t0
+        def work(i):
+            if time.perf_counter() > deadline:
+                return None
+            r = oracle.solve(problems[i], eps_abs=tol, eps_rel=tol, feas_cap=tol, threads=1,
+                             hist_cap=1)
+            return r.status
+
+        with ThreadPoolExecutor(threads) as ex:
+            for st in ex.map(work, range(len(problems))):
+                if st is not None:
+                    done.append(st)
+        return done, time.perf_counter() - t0
+
+    done, el = rate(cores, budget_s)
+    one, el1 = rate(1, budget_s / 4)
+    ko = small_lp(0)
+
+    def per_solve(threads):
+        oracle.solve(ko.problem, threads=threads, hist_cap=1)
+        ts, it = [], 0
+        for _ in range(3):
+            t = time.perf_counter()
+            r = oracle.solve(ko.problem, threads=threads, hist_cap=1)
+            ts.append(time.perf_counter() - t)
+            it = r.iterations
+        return float(np.median(ts)), it, abs(r.objective - ko.f_star) / max(1.0, abs(ko.f_star))
+
+    c0_all, it0, err0 = per_solve(cores)
+    c0_one, _, _ = per_solve(1)
     return dict(value=len(done) / el, unit="LPs/s", cores=cores, kind="port",
                 sample=f"first {len(done)} LPs of the batch, FP64 C restatement of the "
-                       f"reference PDLP, one LP per thread, {el:.1f} s wall",
-                optimal=int(sum(1 for s in done if s == 0)))
+                       f"reference PDLP, one LP per thread on {cores} threads, {el:.1f} s wall",
+                optimal=int(sum(1 for s in done if s == 0)),
+                affinity_cores=affinity, thread_cap=cap,
+                one_thread={"value": len(one) / el1, "unit": "LPs/s", "lps": len(one),
+                            "seconds": el1},
+                configs0={"lp": "500 x 2000 (250 eq, 250 <=), 5 nnz per column, seed 0",
+                          "tolerance": 1e-8, "iterations": it0, "rel_obj_err": err0,
+                          "seconds_per_solve_median_of_3": c0_all, "threads": cores,
+                          "seconds_per_solve_one_thread": c0_one})
+
+
+def gpu_configs0(reps: int = 3) -> dict:
+    """The 500 x 2000 LP on the GPU's single-LP engine (FP64, one workgroup, the
+    reference's arithmetic) at rel KKT 1e-8: wall time per solve with the
+    inputs resident, median of ``reps`` after a warm-up."""
+    import numpy as np
+    import torch
+    from sovlp.lp.pdlp import DeviceLP, Engine, PDLPParams
+    from sovlp.workloads import small_lp
+    ko = small_lp(0)
+    par = PDLPParams(device="gpu", precision=64, team="block")
+    dlp = DeviceLP(ko.problem, 64)
+    ts, it = [], 0
+    for k in range(reps + 1):
+        eng = Engine(dlp, par, history_cap=1)
+        try:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            while eng.info().stage != 3:
+                eng.advance(par.chunk_iters, par.time_limit)
+            dt = time.perf_counter() - t
+            inf = eng.info()
+        finally:
+            eng.close()
+        if k:
+            ts.append(dt)
+        it = int(inf.iterations)
+    return {"seconds_per_solve_median": float(np.median(ts)), "iterations": it,
+            "rel_obj_err": abs(inf.objective - ko.f_star) / max(1.0, abs(ko.f_star))}
+
+
+def large_lp_section(ko, locality: bool = False) -> dict:
+    """The 200k x 1M skewed LP solved from scratch: FP64 (the reference's
+    arithmetic) and FP32 + iterative refinement, rel KKT 1e-6 (feas_cap 1e-6);
+    time on the device clock and the wall, iterations, objective error against
+    the closed form, and for FP32 the refinement record."""
+    from sovlp.lp.pdlp import PDLPParams, solve_pdlp
+    f = ko.f_star
+    rel = lambda v: abs(v - f) / max(1.0, abs(f))  # noqa: E731
+    out = {"lp": f"{ko.problem.m} x {ko.problem.n}, {ko.problem.A.nnz} nonzeros, Zipf column "
+                 f"degrees (s=1.3, 2..5000)", "tolerance": 1e-6}
+    orders = ["given"] + (["locality"] if locality else [])
+    for order in orders:
+        for prec in (64, 32):
+            par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6, device="gpu",
+                             precision=prec, time_limit=240.0, column_order=order)
+            t = time.perf_counter()
+            sol = solve_pdlp(ko.problem, par)
+            wall = time.perf_counter() - t
+            r = {"status": sol.status.name, "iterations": int(sol.iterations),
+                 "restarts": int(sol.info.restarts), "device_s": sol.info.device_seconds,
+                 "wall_s": wall, "us_per_iteration": sol.info.device_seconds /
+                 max(1, sol.iterations) * 1e6, "workgroups": sol.info.workgroups,
+                 "rel_obj_err": rel(sol.objective)}
+            if sol.info.refinement:
+                r["refinement"] = [{"iterations": int(q[0]), "primal_res": q[1], "dual_res": q[2],
+                                    "gap": q[3], "rel_obj_err": rel(q[4])}
+                                   for q in sol.info.refinement]
+            key = ("fp64" if prec == 64 else "fp32_refined") + \
+                ("" if order == "given" else "_locality")
+            out[key] = r
+            log(f"large LP {key}: {r}")
+    return out
+
+
+def serve_section(a) -> dict:
+    """A Poisson stream of blending LPs (K 20-60, P 5-20, T in the given range,
+    30 % warm-started 1 % resolves) through the serving loop, admission by
+    free HBM: p50/p99 latency, LPs/s, outcomes."""
+    import numpy as np
+    import torch
+    from sovlp.lp.pdlp import PDLPParams
+    from sovlp.lp.serve import (DeviceServer, HbmAdmission, ServeLoop, hbm_budget,
+                                poisson_blending_requests, summarize)
+    t0 = time.perf_counter()
+    reqs = poisson_blending_requests(a.serve_rate, a.serve_duration, 0, 0.3, a.serve_tmin,
+                                     a.serve_tmax)
+    nnz = np.array([r.problem.A.nnz for r in reqs])
+    log(f"serve: {len(reqs)} requests built in {time.perf_counter() - t0:.1f} s, nnz mean "
+        f"{nnz.mean():.0f} max {nnz.max()}")
+    tol = a.tol
+    mk = lambda prec: PDLPParams(eps_abs=tol, eps_rel=tol, feas_cap=tol, device="gpu",  # noqa
+                                 precision=prec, time_limit=60.0)
+    srv = DeviceServer(a.serve_slots, mk(a.serve_precision), precision=a.serve_precision,
+                       phases_per_launch=32, team=a.serve_team)
+    fb = DeviceServer(32, mk(64), precision=64, phases_per_launch=32, team=a.serve_team) \
+        if a.serve_precision == 32 else None
+    adm = HbmAdmission(hbm_budget(0.8))
+    try:
+        ServeLoop(srv, HbmAdmission(adm.budget)).run(reqs[:1])     # warm-up
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out = ServeLoop(srv, adm, fallback=fb,
+                        progress=lambda d: log(f"serve: {d}")).run(reqs)
+        wall = time.perf_counter() - t1
+    finally:
+        srv.close()
+        if fb is not None:
+            fb.close()
+    st = summarize(out)
+    st.update(dict(workload=f"Poisson {a.serve_rate:g} req/s for {a.serve_duration:g} s, "
+                            f"blending K 20-60, P 5-20, T {a.serve_tmin}-{a.serve_tmax}, "
+                            f"30% warm-started resolves",
+                   wall_s=wall, nnz_mean=float(nnz.mean()), nnz_max=int(nnz.max()),
+                   hbm_budget_bytes=adm.budget, hbm_peak_bytes=adm.peak, slots=a.serve_slots,
+                   team=a.serve_team, precision=a.serve_precision, tolerance=tol))
+    return st
+
+
+def copy_calibration(mib: int = 256, reps: int = 3) -> dict:
+    """A plain device-to-device copy of a known size: rocprofv3's FETCH_SIZE and
+    WRITE_SIZE read on it give the factor between what the counters report and
+    the bytes moved, which then scales the solver kernels' readings."""
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"copy_bytes_read": n, "copy_bytes_written": n, "copies": reps}
 
 
 def main():
@@ -321,7 +512,34 @@ def main():
     from sovlp.lp.batch import BatchEngine
     from sovlp.lp.pdlp import PDLPParams
     from sovlp.lp.traffic import bytes_of_counts
-    from sovlp.workloads import uneven_batch
+    from sovlp.workloads import skewed_lp, uneven_batch
+
+    big = {}
+
+    def large():
+        """The 200k x 1M skewed LP, built once."""
+        if "ko" not in big:
+            t = time.perf_counter()
+            big["ko"] = skewed_lp(1)
+            log(f"200k x 1M LP built in {time.perf_counter() - t:.1f} s")
+        return big["ko"]
+
+    def spmv_section():
+        from sovlp.spmv_bench import skewed_roofline
+        res, _ = skewed_roofline(A=large().problem.A, panels=a.panels)
+        for r in res:
+            log(f"spmv {r['op']} {r['kernel']} FP{r['precision']}: {r['ms'] * 1e3:.1f} us, "
+                f"{r['gbs']:.0f} GB/s ({r['frac']:.1%})")
+        return {"lp": "200000 x 1000000 skewed LP", "peak_gbs": HBM_PEAK_GBS,
+                "bytes_model": "nnz (s_val + 4) + (rows + 1) 4 + cols s_val + rows s_val",
+                "results": res}
+
+    if a.only:
+        sec = {"spmv": spmv_section, "large_lp": lambda: large_lp_section(
+            large(), a.large_lp_locality), "bnr": bnr_rate, "certify": lambda: certify_rate(k=large()),
+               "serve": lambda: serve_section(a), "copy": copy_calibration}[a.only]()
+        print(json.dumps({a.only: sec}), flush=True)
+        return
 
     t = time.perf_counter()
     kos = uneven_batch(a.lps, base_seed=a.seed)
@@ -461,15 +679,20 @@ def main():
         cb = cpu_baseline(problems, a.tol, a.cpu_seconds)
         log(f"cpu baseline: {cb}")
         out["cpu_baseline"] = cb
-    if a.spmv:
-        from sovlp.spmv_bench import skewed_roofline
-        res, gen = skewed_roofline()
-        out["spmv"] = res
+    out["gpu_configs0"] = gpu_configs0()
+    log(f"500 x 2000 LP on the GPU: {out['gpu_configs0']}")
+    if not a.no_spmv:
+        out["spmv"] = spmv_section()
+    if not a.no_large_lp:
+        out["large_lp"] = large_lp_section(large(), a.large_lp_locality)
+    if a.serve:
+        out["serve"] = serve_section(a)
+        log(f"serve: {out['serve']}")
     if a.bnr:
         out["bnr"] = bnr_rate()
         log(f"bnr: {out['bnr']}")
     if a.certify:
-        out["certify"] = certify_rate()
+        out["certify"] = certify_rate(k=large())
         log(f"certificate: {out['certify']}")
     print(json.dumps(out), flush=True)
 

@@ -291,7 +291,7 @@ typedef struct sovlp_pdlp sovlp_pdlp;
 
 /* Prepares a solve in the caller's workspace (>= workspace_bytes) and records
  * one replay of it into a hipGraph.  `team` 0 = choose, 1 = one workgroup
- * (a replay is one launch running up to 128 phases of the solve back to
+ * (a replay is one launch running up to sovlp_phases_per_launch() phases of the solve back to
  * back), 2 = W workgroups per phase (W from the tile count; a replay is 64
  * phase launches, each followed by a one-workgroup reduction launch).
  * Synchronises. */
@@ -300,8 +300,10 @@ int sovlp_pdlp_create(const sovlp_lp *lp, const sovlp_params *params, int precis
                       sovlp_stream_t stream, sovlp_pdlp **out);
 
 /* Replays the graph until the solve has advanced `iteration_budget`
- * iterations or terminated.  `seconds_left` is the device time budget of the
- * solve; it is taken on the first call.  Synchronises. */
+ * iterations (counted over every refinement round) or terminated; with one
+ * workgroup the device stops at the budget exactly, with W workgroups within
+ * one replay of it.  `seconds_left` is the device time budget of the solve; it
+ * is taken on the first call.  Synchronises. */
 int sovlp_pdlp_advance(sovlp_pdlp *h, int64_t iteration_budget, double seconds_left,
                        sovlp_stream_t stream);
 
@@ -319,6 +321,19 @@ int sovlp_pdlp_refinement(sovlp_pdlp *h, double **rows);
 
 /* Workgroups per phase chosen by sovlp_pdlp_create. */
 int sovlp_pdlp_width(sovlp_pdlp *h, int *workgroups);
+
+/* Device pointers of the scaling factors (FP64, m row and n column factors,
+ * exact powers of two), valid once the solve has passed its scaling phases:
+ * the row and column factors of scale_problem(prob, "pdlp"),
+ * src/sovopt/numerics/scaling.py:374-478 (the objective factor is
+ * sovlp_info.scale_obj). */
+int sovlp_pdlp_scaling(sovlp_pdlp *h, double **row, double **col);
+
+/* Phases a launch of the run kernel advances each LP by, at most (the
+ * batched engine, the one-workgroup single-LP engine): a replay count that
+ * bounds a solve of max_iter iterations is (61 max_iter + 400) / phases + 16,
+ * an iteration taking at most 61 phases. */
+int sovlp_phases_per_launch(int *phases);
 
 int sovlp_pdlp_destroy(sovlp_pdlp *h);
 
@@ -339,7 +354,7 @@ int sovlp_pdlp_phase_counts(sovlp_pdlp *h, int64_t *counts, sovlp_stream_t strea
  * (pdlp.py:238); its only batched PDHG shares one A across B&B nodes
  * (src/sovopt/mip/bnr.py:176-301).  Here each LP keeps its own matrix,
  * scaling, step size, restarts and termination.  Every launch advances every
- * unfinished LP by up to 128 phases, one workgroup per LP, and the launch is
+ * unfinished LP by up to sovlp_phases_per_launch() phases, one workgroup per LP, and the launch is
  * replayed from a hipGraph, so the arithmetic of an LP does not depend on the
  * other LPs in the batch.
  * Tile plans must use sovlp_batch_geometry's tile size and row count. */

@@ -23,8 +23,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_pdlp.so")
+# the same source with fused multiply-adds (rounding study only)
+FMA_LIB_PATH = os.path.join(HERE, "liboracle_pdlp_fma.so")
 
-_lib = None
+_libs: dict = {}
 
 
 class _Params(C.Structure):
@@ -44,20 +46,22 @@ class _Info(C.Structure):
                 ("scale_obj", C.c_double), ("history_len", C.c_longlong)]
 
 
-def build(force: bool = False) -> str:
+def build(force: bool = False, fma: bool = False) -> str:
+    path = FMA_LIB_PATH if fma else LIB_PATH
     src = max(os.path.getmtime(os.path.join(HERE, f)) for f in ("pdlp_oracle.c", "certify_oracle.c"))
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < src:
-        subprocess.run(["make", "-C", HERE, "-B" if force else "liboracle_pdlp.so"],
+    if force or not os.path.exists(path) or os.path.getmtime(path) < src:
+        subprocess.run(["make", "-C", HERE] + (["-B"] if force else []) + [os.path.basename(path)],
                        check=True, capture_output=True)
-    return LIB_PATH
+    return path
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = C.CDLL(LIB_PATH)
+def lib(fma: bool = False):
+    """The oracle library; ``fma``: the build with contracted multiply-adds."""
+    if fma not in _libs:
+        path = FMA_LIB_PATH if fma else LIB_PATH
+        if not os.path.exists(path):
+            build(fma=fma)
+        L = C.CDLL(path)
         P = C.c_void_p
         L.oracle_pdlp_solve.restype = C.c_int
         L.oracle_pdlp_solve.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P,
@@ -73,8 +77,8 @@ def lib():
         L.oracle_certify.argtypes = [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P,
                                      C.c_double, C.c_int, P, P, C.c_double, P, P, P, P, P,
                                      C.POINTER(_Certificate)]
-        _lib = L
-    return _lib
+        _libs[fma] = L
+    return _libs[fma]
 
 
 def _p(a):
@@ -159,7 +163,10 @@ def power_start(n: int) -> np.ndarray:
 
 
 def solve(prob, eps_abs=1e-8, eps_rel=1e-8, feas_cap=1e-6, max_iter=200_000, time_limit=600.0,
-          check_every=64, stepsize_every=8, threads=0, hist_cap=100_000) -> OracleResult:
+          check_every=64, stepsize_every=8, threads=0, hist_cap=100_000,
+          fma: bool = False) -> OracleResult:
+    """``fma``: run the build with contracted multiply-adds (same algorithm, other
+    rounding), to measure how rounding alone moves the trajectory."""
     A = prob.A
     m, n = A.m, A.n
     par = _Params(eps_abs, eps_rel, feas_cap, max_iter, time_limit, check_every, stepsize_every,
@@ -172,7 +179,7 @@ def solve(prob, eps_abs=1e-8, eps_rel=1e-8, feas_cap=1e-6, max_iter=200_000, tim
     arrs = [np.ascontiguousarray(a) for a in (A.cp, A.ci, A.cx, A.rp, A.ri, A.rx, prob.c,
                                               prob.row_lb, prob.row_ub, prob.col_lb, prob.col_ub)]
     v0 = power_start(n)
-    rc = lib().oracle_pdlp_solve(m, n, *[_p(a) for a in arrs], float(prob.obj_offset),
+    rc = lib(fma).oracle_pdlp_solve(m, n, *[_p(a) for a in arrs], float(prob.obj_offset),
                                  int(int(prob.sense) == -1), _p(v0), C.byref(par), _p(x), _p(y),
                                  _p(d), C.byref(info), _p(hist), hist_cap, threads)
     if rc != 0:

@@ -26,6 +26,8 @@ struct sovlp_pdlp {
   hipStream_t cap;
   long long ticks_set;
   LpState* hstate;           // [2] pinned host copies of the state
+  long long* hkstop;         // pinned: the iteration stop of the next replays
+  double *rowf, *colf;       // scaling factors (device, FP64)
   hipEvent_t ev[2];
 };
 
@@ -217,6 +219,7 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   s0.phase = PH_INIT;
   s0.status = ST_RUNNING;
   StepCtl c0{};
+  c0.kstop = kNoStop;
   std::memcpy(&c0.P, prm, sizeof(Params));
   c0.ticks = (long long)9.0e18;
   HIPCHK(hipMemcpyAsync(L.st, &s0, sizeof s0, hipMemcpyHostToDevice, st));
@@ -235,6 +238,8 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   h->ctl = ctl;
   h->done = done;
   h->xo = L.xo; h->yo = L.yo; h->dout = L.dout; h->hist = L.hist; h->rhist = L.rhist;
+  h->rowf = L.rowf;
+  h->colf = L.colf;
   h->pcount = pcount;
   h->hist_cap = hist_cap;
   h->ticks_set = c0.ticks;
@@ -257,6 +262,7 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
   HIPCHK(hipHostMalloc((void**)&h->hstate, 2 * sizeof(LpState), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&h->hkstop, sizeof(long long), hipHostMallocDefault));
   for (hipEvent_t& e : h->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return clock_hz(&h->hz);
 }
@@ -320,6 +326,7 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, cons
     h->rhist[i] = L.rhist;
   }
   StepCtl c0{};
+  c0.kstop = kNoStop;
   std::memcpy(&c0.P, prm, sizeof(Params));
   c0.ticks = (long long)9.0e18;
   HIPCHK(hipMemcpyAsync(d_lps, host.data(), sizeof(LpDev<T>) * count, hipMemcpyHostToDevice, st));
@@ -498,6 +505,7 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
     h->hinit[i].status = ST_RUNNING;
   }
   StepCtl c0{};
+  c0.kstop = kNoStop;
   std::memcpy(&c0.P, prm, sizeof(Params));
   const double secs = prm->time_limit > 0.0 ? prm->time_limit : 0.0;
   double hz = 0.0;

@@ -79,11 +79,18 @@ int sovlp_pdlp_advance(sovlp_pdlp* h, int64_t iteration_budget, double seconds_l
   }
   HIPCHK(hipMemcpyAsync(&h->hstate[0], h->st, sizeof(LpState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  const long long k0 = h->hstate[0].k;
+  // iterations of every refinement round so far
+  const long long k0 = h->hstate[0].k_done + h->hstate[0].k;
   const long long budget = iteration_budget < 1 ? 1 : iteration_budget;
-  // every replay runs phases_per_replay phases; an iteration takes at most 61
-  // phases (60 rejected trials), so this bound is never the one that stops a
-  // healthy solve
+  if (h->W == 1) {
+    // the run kernel leaves the LP once it has run the budget, so that a
+    // replay of up to kPhasesPerLaunch phases does not overshoot it
+    *h->hkstop = k0 + budget;
+    HIPCHK(hipMemcpyAsync(&h->ctl->kstop, h->hkstop, sizeof(long long), hipMemcpyHostToDevice, st));
+  }
+  // every replay runs up to phases_per_replay phases; an iteration takes at
+  // most 61 phases (60 rejected trials), so this bound is never the one that
+  // stops a healthy solve
   const long long max_replays = (budget * 61 + 200) / h->phases_per_replay + 16;
   // one replay stays queued while the host reads the state the previous one
   // left (pinned copy + event), so the device does not idle on the host
@@ -105,7 +112,7 @@ int sovlp_pdlp_advance(sovlp_pdlp* h, int64_t iteration_budget, double seconds_l
     }
     HIPCHK(hipEventSynchronize(h->ev[cur]));
     const LpState& s = h->hstate[cur];
-    if (s.phase == PH_DONE || s.k - k0 >= budget || !more) break;
+    if (s.phase == PH_DONE || s.k_done + s.k - k0 >= budget || !more) break;
     cur = nxt;
   }
   HIPCHK(hipStreamSynchronize(st));
@@ -133,6 +140,19 @@ int sovlp_pdlp_outputs(sovlp_pdlp* h, double** x, double** y, double** d, double
 int sovlp_pdlp_refinement(sovlp_pdlp* h, double** rows) {
   if (!h || !rows) return fail(1, "null argument");
   *rows = h->rhist;
+  return 0;
+}
+
+int sovlp_pdlp_scaling(sovlp_pdlp* h, double** row, double** col) {
+  if (!h) return fail(1, "null handle");
+  if (row) *row = h->rowf;
+  if (col) *col = h->colf;
+  return 0;
+}
+
+int sovlp_phases_per_launch(int* phases) {
+  if (!phases) return fail(1, "null argument");
+  *phases = kPhasesPerLaunch;
   return 0;
 }
 
@@ -171,6 +191,7 @@ int sovlp_pdlp_destroy(sovlp_pdlp* h) {
   for (hipEvent_t e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->hstate) (void)hipHostFree(h->hstate);
+  if (h->hkstop) (void)hipHostFree(h->hkstop);
   if (h->exec) (void)hipGraphExecDestroy(h->exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->cap) (void)hipStreamDestroy(h->cap);

@@ -20,9 +20,13 @@ constexpr int lds_doubles() {
   return sizeof(T) == 8 ? 2 * TNNZ : TNNZ;
 }
 
+constexpr long long kNoStop = 0x7fffffffffffffffLL;
+
 struct StepCtl {             // device-resident control block (read by every launch)
   Params P;
   long long ticks;           // time budget in device clock ticks, taken when a solve starts
+  long long kstop;           // a run launch leaves an LP once it has run this many
+                             // iterations (all refinement rounds); kNoStop: never
 };
 
 __device__ __forceinline__ void init_part(int ph, double (&part)[kMaxRed]) {
@@ -104,11 +108,13 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   __shared__ LpState ss;
   __shared__ Params sp;                  // the parameters, read once per launch
+  __shared__ long long kstop;
   __shared__ int pc[kPhaseSlots];        // phase counts of this launch
   const LpDev<T>& L = lps[blockIdx.x];
   if (threadIdx.x == 0) {
     ss = *L.st;
     sp = ctl->P;
+    kstop = ctl->kstop;
   }
   if (threadIdx.x < kPhaseSlots) pc[threadIdx.x] = 0;
   __syncthreads();
@@ -116,6 +122,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
   for (int it = 0; it < max_phases; ++it) {
     const int ph = uniform(ss.phase);
     if (ph == PH_DONE) break;
+    if (ss.k_done + ss.k >= kstop) break;   // the caller's iteration budget
     // the fused A phase reduces nothing, and the fused B phase only ||dx||^2
     // for a step-size trial that follows it.  Read here, while no thread
     // writes the state: thread 0 changes it right after the phase.

@@ -32,6 +32,13 @@ from .pdlp import PDLPInfo, PDLPParams, plan_tiles, refinement_rows
 __all__ = ["BatchResult", "BatchEngine", "solve_batch", "DeviceBatch", "batch_geometry"]
 
 
+def phases_per_launch() -> int:
+    """Phases a run-kernel launch advances each LP by, at most."""
+    v = C.c_int()
+    _lib.call("sovlp_phases_per_launch", C.byref(v))
+    return v.value
+
+
 def batch_geometry(precision: int, block: int) -> tuple[int, int]:
     r, t = C.c_int(), C.c_int()
     _lib.call("sovlp_batch_geometry", int(precision), int(block), C.byref(r), C.byref(t))
@@ -147,7 +154,9 @@ class BatchEngine:
                   self.precision, self.db.blocks.ctypes.data, self.ws.data_ptr(), nbytes.value,
                   int(history_cap), self.stream, C.byref(h))
         self.h = h
-        self.max_graphs = (int(self.params.max_iter) * 61 + 400) // 64 + 16
+        # a bound on the replays of one solve (an iteration takes at most 61
+        # phases; the device enforces the iteration and time limits itself)
+        self.max_graphs = (int(self.params.max_iter) * 61 + 400) // phases_per_launch() + 16
 
     def run(self) -> int:
         """Solves the whole batch (synchronous); returns the graph replays."""

@@ -249,6 +249,17 @@ class Engine:
         return self._view(x.value, n), self._view(y.value, m), self._view(d.value, n), \
             self._view(h.value, 5 * self.hist_cap)
 
+    def scaling(self) -> tuple[np.ndarray, np.ndarray]:
+        """Row and column factors of the device's Ruiz + Pock-Chambolle scaling
+        (FP64 powers of two, as ``scale_problem(prob, "pdlp")`` returns them,
+        reference ``numerics/scaling.py:374-478``), once the solve has passed
+        its scaling phases."""
+        r, c = C.c_void_p(), C.c_void_p()
+        _lib.call("sovlp_pdlp_scaling", self.h, C.byref(r), C.byref(c))
+        m, n = self.dlp.desc.m, self.dlp.desc.n
+        return (self._view(r.value, m).cpu().numpy().copy(),
+                self._view(c.value, n).cpu().numpy().copy())
+
     def refinement(self, rows: int) -> list:
         """The refinement record (``rows`` rows, as ``sovlp_pdlp_refinement``)."""
         if rows <= 0:

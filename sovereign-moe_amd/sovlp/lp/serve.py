@@ -42,7 +42,7 @@ from .batch import batch_geometry
 from .pdlp import DeviceLP, PDLPInfo, PDLPParams, refinement_rows
 
 __all__ = ["Request", "Served", "HbmAdmission", "hbm_budget", "DeviceServer", "ServeLoop",
-           "summarize", "WarmStart", "resolve_warm"]
+           "summarize", "WarmStart", "resolve_warm", "poisson_blending_requests"]
 
 
 @dataclass
@@ -428,3 +428,33 @@ def resolve_warm(device, problems: list[Problem], warm: list[WarmStart],
         reqs.append(Request(i, p, 0.0, model=i, resolve=True))
     out = {s.key: s for s in loop.run(reqs)}
     return [out[i].solution for i in range(len(problems))]
+
+
+def poisson_blending_requests(rate: float, duration: float, seed: int, resolve: float, tmin: int,
+                              tmax: int, kmax: int = 60, pmax: int = 20) -> list[Request]:
+    """A seeded Poisson stream of blending requests, every model built up front:
+    arrivals at ``rate`` per second for ``duration`` seconds; a fraction
+    ``resolve`` re-solves an earlier model with ``b`` and ``c`` perturbed by 1 %
+    (warm-startable); the rest are new models with ``K ~ U{20..kmax}``,
+    ``P ~ U{5..pmax}``, ``T ~ U{tmin..tmax}``."""
+    from ..workloads import blending_lp, perturbed
+    rng = np.random.default_rng(seed)
+    t, key = 0.0, 0
+    reqs, models = [], {}
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        if models and rng.random() < resolve:
+            mk = int(rng.choice(list(models)))
+            p = perturbed(models[mk], int(rng.integers(1 << 30)), 0.01)
+            reqs.append(Request(key, p, t, model=mk, resolve=True))
+        else:
+            K = int(rng.integers(20, kmax + 1))
+            P = int(rng.integers(5, pmax + 1))
+            T = int(rng.integers(tmin, tmax + 1))
+            p = blending_lp(int(rng.integers(1 << 30)), K=K, P=P, T=T)
+            models[key] = p
+            reqs.append(Request(key, p, t, model=key))
+        key += 1
+    return reqs

@@ -168,11 +168,14 @@ def measure_panels(ptr, idx, val, ncols: int, precision: int = 64, panels: int =
                 frac=b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
 
 
-def skewed_roofline(reps: int = 50, seed: int = 1):
-    """SpMV roofline of A and Aᵀ of the 200k x 1M skewed LP, FP64 and FP32."""
+def skewed_roofline(reps: int = 50, seed: int = 1, A=None, panels: bool = False):
+    """SpMV roofline of A and Aᵀ of the 200k x 1M skewed LP (``A``: its matrix,
+    built from ``seed`` when not given), FP64 and FP32, in the given column
+    order and in locality order; ``panels``: also A x as eight column panels."""
     from .workloads import skewed_lp
     t = time.perf_counter()
-    A = skewed_lp(seed).problem.A
+    if A is None:
+        A = skewed_lp(seed).problem.A
     gen_s = time.perf_counter() - t
     B = relabel_columns(A, locality_order(A))
     out = []
@@ -180,15 +183,10 @@ def skewed_roofline(reps: int = 50, seed: int = 1):
         out.append(dict(op="A*x", kernel="tiled", **measure(A.rp, A.ri, A.rx, A.n, prec, reps)))
         out.append(dict(op="A*x", kernel="tiled, columns in locality order",
                         **measure(B.rp, B.ri, B.rx, B.n, prec, reps)))
-        out.append(dict(op="A*x", kernel="8 column panels",
-                        **measure_panels(A.rp, A.ri, A.rx, A.n, prec, 8, reps=reps)))
+        if panels:
+            out.append(dict(op="A*x", kernel="8 column panels",
+                            **measure_panels(A.rp, A.ri, A.rx, A.n, prec, 8, reps=reps)))
         out.append(dict(op="At*y", kernel="tiled", **measure(A.cp, A.ci, A.cx, A.m, prec, reps)))
         out.append(dict(op="At*y", kernel="tiled, columns in locality order",
                         **measure(B.cp, B.ci, B.cx, B.m, prec, reps)))
     return out, gen_s
-
-
-if __name__ == "__main__":
-    import json
-    res, gen = skewed_roofline()
-    print(json.dumps(dict(gen_s=gen, results=res), indent=1))

@@ -1,13 +1,16 @@
 """Write golden objectives of small multi-period blending LPs with SciPy's HiGHS.
 
-    python tests/golden/make_blending_vectors.py
+    python tests/golden/make_blending_vectors.py           (small instances)
+    python tests/golden/make_blending_vectors.py --spec    (served-workload sizes)
 
 The blending LPs (``sovlp.workloads.blending_lp``) and their 1 % resolves
 (``sovlp.workloads.perturbed``) have no closed-form optimum, so an independent
 LP solver pins them: HiGHS (dual simplex / IPM with crossover, SciPy 1.15) solves
 each instance here and the objective is stored. The output,
 ``blending_highs.json``, holds the generator arguments and the objectives only;
-the tests rebuild each LP from its seed.
+the tests rebuild each LP from its seed. ``--spec`` writes
+``blending_spec_highs.json``: instances inside the served workload's ranges
+(K >= 20 components, P >= 5 products, T = 100-120 hours) and a 1 % resolve.
 """
 
 from __future__ import annotations
@@ -50,6 +53,14 @@ def cases():
     return out
 
 
+def spec_cases():
+    """Instances inside the served workload's ranges (K 20-60, P 5-20, T 24-720)
+    that PDLP finishes in seconds on one GPU, and a 1 % resolve of the first."""
+    return [dict(seed=11, K=20, P=5, T=100, perturb_seed=None),
+            dict(seed=11, K=20, P=5, T=100, perturb_seed=311),
+            dict(seed=13, K=22, P=6, T=120, perturb_seed=None)]
+
+
 def build(case):
     p = blending_lp(case["seed"], K=case["K"], P=case["P"], T=case["T"])
     if case["perturb_seed"] is not None:
@@ -58,13 +69,15 @@ def build(case):
 
 
 def main():
+    spec = "--spec" in sys.argv[1:]
     res = []
-    for c in cases():
+    for c in (spec_cases() if spec else cases()):
         p = build(c)
         c = dict(c, m=p.m, n=p.n, nnz=int(p.A.nnz), objective=highs_objective(p))
         res.append(c)
-        print(c)
-    with open(os.path.join(HERE, "blending_highs.json"), "w") as f:
+        print(c, flush=True)
+    out = "blending_spec_highs.json" if spec else "blending_highs.json"
+    with open(os.path.join(HERE, out), "w") as f:
         json.dump({"solver": "scipy.optimize.linprog(method='highs')", "cases": res}, f, indent=1)
 
 

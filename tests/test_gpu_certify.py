@@ -176,3 +176,46 @@ def test_verify_accepts_a_gpu_pdlp_optimum_and_separates_the_other_cases():
     b, pert = cert.certified_bound(p, s.x, s.y)
     assert pert == 0.0 and b <= k.f_star + 1e-9 * abs(k.f_star)
     assert k.f_star - b <= 1e-6 * abs(k.f_star)
+
+
+def test_long_lines_listed_over_chunks_of_many_rounds():
+    """More than 262,144 lines (1024 workgroups x 256 lanes), so each workgroup
+    of the listing counts and writes its chunk in several rounds: long rows
+    (and columns) scattered over the chunks, some adjacent, one at each end.
+    The list comes out in index order; the compensated residual and the
+    certificate are the oracle's bit for bit."""
+    import ctypes
+
+    import torch
+    from sovlp import _lib
+    print("[gpu] certify: 300k-line listing", flush=True)
+    rng = np.random.default_rng(21)
+    m, n = 300_000, 4_000
+    lens = rng.integers(0, 6, m)
+    longs = np.unique(np.concatenate([[0, 1, 2, m - 1], rng.choice(m, 60, replace=False),
+                                      np.arange(150_000, 150_005)]))
+    lens[longs] = rng.integers(129, 3000, longs.size)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ri = rng.integers(0, n, rp[-1]).astype(np.int32)
+    rx = rng.standard_normal(rp[-1]) * 10.0 ** rng.integers(-4, 5, rp[-1])
+    x = rng.standard_normal(n)
+    b = rng.standard_normal(m)
+    dev = [torch.from_numpy(a).cuda() for a in (rp, ri, rx, x, b)]
+    out = torch.empty(m, dtype=torch.float64, device="cuda")
+    nb = ctypes.c_int64()
+    _lib.call("sovlp_residual_workspace_bytes", m, ctypes.byref(nb))
+    ws = torch.zeros(int(nb.value) // 4, dtype=torch.int32, device="cuda")
+    _lib.call("sovlp_residual_f64", m, *[a.data_ptr() for a in dev], out.data_ptr(),
+              ws.data_ptr(), int(nb.value), torch.cuda.current_stream().cuda_stream)
+    w = ws.cpu().numpy()
+    assert w[0] == longs.size
+    assert np.array_equal(w[4: 4 + longs.size], longs)
+    _same_bits(out.cpu().numpy(), oracle.residual(rp, ri, rx, x, b))
+    # the certificate of an LP whose CSC has 300k columns, long ones scattered
+    # (the transpose of the residual's matrix: its long rows become long columns)
+    A = SparseMatrix.from_triplets(ri.astype(np.int64), np.repeat(np.arange(m), lens), rx, n, m)
+    prob = Problem(A=A, c=rng.standard_normal(m), row_lb=np.full(n, -1.0), row_ub=np.ones(n),
+                   col_lb=np.zeros(m), col_ub=np.full(m, 5.0))
+    xs = rng.uniform(0.0, 5.0, m)
+    ys = rng.standard_normal(n)
+    _check_against_oracle(prob, xs, ys)
