@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(BLOCK)
                       const T* __restrict__ val, const int32_t* __restrict__ tiles,
                       const int32_t* __restrict__ part, const T* __restrict__ x,
                       T* __restrict__ y) {
-  __shared__ __align__(16) T lds[TNNZ];
+  __shared__ __align__(16) T lds[tile_lds_elems<BLOCK, TNNZ, 1>()];
   const int t0 = part[blockIdx.x], t1 = part[blockIdx.x + 1];
   auto prod = [&](int q, int j, T (&v)[1]) { v[0] = val[q] * x[j]; };
   auto epi = [&](int r, const T (&s)[1]) { y[r] = s[0]; };
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(BLOCK)
                       const int32_t* __restrict__ tiles, const int32_t* __restrict__ tile_off,
                       const int32_t* __restrict__ parts, int groups, const T* __restrict__ x,
                       T* __restrict__ partial) {
-  __shared__ __align__(16) T lds[TNNZ];
+  __shared__ __align__(16) T lds[tile_lds_elems<BLOCK, TNNZ, 1>()];
   const int p = (int)blockIdx.x % panels, g = (int)blockIdx.x / panels;
   const int32_t* pp = parts + (size_t)p * (groups + 1);
   const int t0 = pp[g], t1 = pp[g + 1];

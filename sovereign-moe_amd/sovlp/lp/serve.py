@@ -291,8 +291,8 @@ class ServeLoop:
 
     ``fallback`` (optional) is a second device, normally an FP64 server: a request
     the first device ends NUMERICAL or at the iteration limit is queued there,
-    warm-started from the first device's ``(x, y)`` and primal weight, and its
-    latency runs until the fallback's result."""
+    from a cold start (``escalate_warm``: from the first device's ``(x, y)`` and
+    primal weight), and its latency runs until the fallback's result."""
     device: object
     admission: HbmAdmission
     launches_per_step: int = 1
@@ -300,6 +300,10 @@ class ServeLoop:
     sleep: object = time.sleep
     warm_starts: bool = True
     fallback: object = None
+    # an escalated request restarts cold on the fallback: the point the first
+    # device ended NUMERICAL at is a poor start (FP64 PDLP from the FP32 point of
+    # a 22 x 6 x 120 blending LP hit 200k iterations, against 121k cold)
+    escalate_warm: bool = False
     warm: dict = field(default_factory=dict)
     progress: object = None       # called with a status dict every progress_every seconds
     progress_every: float = 2.0
@@ -371,7 +375,8 @@ class ServeLoop:
                     if li == 0 and len(lanes) > 1 and st in _ESCALATE and sol.x is not None:
                         s.escalated = True
                         s.first_status = st
-                        w = WarmStart(sol.x, sol.y, getattr(sol.info, "primal_weight", 0.0))
+                        w = WarmStart(sol.x, sol.y, getattr(sol.info, "primal_weight", 0.0)) \
+                            if self.escalate_warm else None
                         lanes[1].queue.append((req, w, s))
                         continue
                     s.solution = sol

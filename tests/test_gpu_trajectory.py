@@ -86,14 +86,19 @@ def test_block_team_run_is_the_reference_run(name):
     assert _agreeing_rows(s.info.history, e["history"], 1e-9) == len(e["history"])
     np.testing.assert_allclose(s.x, e["x"], rtol=1e-9, atol=1e-9)
     np.testing.assert_allclose(s.y, e["y"], rtol=1e-8, atol=1e-8)
-    assert s.info.step_size == pytest.approx(e["step_size"], rel=1e-8)
-    assert s.info.primal_weight == pytest.approx(e["primal_weight"], rel=1e-8)
+    # the final step size comes from the last step-size trial, whose
+    # interaction term dy'A dx is a difference of nearly equal sums near the
+    # optimum: rounding moves it by ~1e-5 relative (the oracle built with fused
+    # multiply-adds lands on the device's value for testprob to 1e-9, and the
+    # plain oracle is 3.5e-6 off the reference's on toy_max)
+    assert s.info.step_size == pytest.approx(e["step_size"], rel=1e-4)
+    assert s.info.primal_weight == pytest.approx(e["primal_weight"], rel=1e-4)
 
 
 # leading history rows (64-iteration checks) that agree with the reference's
 # run to 1e-6 on the device (FP64, one workgroup); deterministic, since the
 # device sums in a fixed order
-AGREE = {"known_12x30_s5": 1, "known_30x80_s6": 1, "known_40x100_s7": 1, "badly_scaled": 1}
+AGREE = {"known_12x30_s5": 6, "known_30x80_s6": 7, "known_40x100_s7": 3, "badly_scaled": 1}
 
 
 @pytest.mark.parametrize("name", sorted(AGREE))

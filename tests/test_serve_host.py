@@ -176,7 +176,8 @@ def test_blending_generator_shape_and_stream():
     assert np.all(np.abs(q.c - p.c) <= 0.01 * np.abs(p.c) + 1e-12)
 
 
-def test_non_optimal_results_escalate_to_the_fallback_warm_started():
+@pytest.mark.parametrize("warm", [False, True])
+def test_non_optimal_results_escalate_to_the_fallback(warm):
     clk = Clock()
 
     class Failing(FakeDevice):
@@ -190,12 +191,12 @@ def test_non_optimal_results_escalate_to_the_fallback_warm_started():
     first = Failing(slots=2, clock=clk)
     second = FakeDevice(slots=1, clock=clk, steps_for=lambda p: 5)
     adm = HbmAdmission(1000)
-    out = ServeLoop(first, adm, clock=clk, sleep=clk.sleep, fallback=second).run(
-        [_req(k, arrival=0.001 * k) for k in range(5)])
+    out = ServeLoop(first, adm, clock=clk, sleep=clk.sleep, fallback=second,
+                    escalate_warm=warm).run([_req(k, arrival=0.001 * k) for k in range(5)])
     by = {s.key: s for s in out}
     assert len(out) == 5 and all(int(s.solution.status) == 0 for s in out)
     assert {k for k, s in by.items() if s.escalated} == {1, 3}
     assert by[1].first_status == int(Status.NUMERICAL)
-    # the fallback got exactly the escalated requests, each warm-started
-    assert [(a[2], a[3]) for a in second.admits] == [("r1", True), ("r3", True)]
+    # the fallback got exactly the escalated requests, cold by default
+    assert [(a[2], a[3]) for a in second.admits] == [("r1", warm), ("r3", warm)]
     assert by[1].finished > by[0].finished and adm.used == 0
