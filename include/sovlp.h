@@ -291,10 +291,10 @@ typedef struct sovlp_pdlp sovlp_pdlp;
 
 /* Prepares a solve in the caller's workspace (>= workspace_bytes) and records
  * one replay of it into a hipGraph.  `team` 0 = choose, 1 = one workgroup
- * (a replay is one launch running up to sovlp_phases_per_launch() phases of the solve back to
- * back), 2 = W workgroups per phase (W from the tile count; a replay is 64
- * phase launches, each followed by a one-workgroup reduction launch).
- * Synchronises. */
+ * (a replay is one launch running up to sovlp_phases_per_launch() phases of
+ * the solve back to back), 2 = W workgroups per phase (W from the tile
+ * count; a replay is 64 phase launches, each followed by a one-workgroup
+ * reduction launch).  Synchronises. */
 int sovlp_pdlp_create(const sovlp_lp *lp, const sovlp_params *params, int precision,
                       void *workspace, int64_t workspace_bytes, int64_t history_cap, int team,
                       sovlp_stream_t stream, sovlp_pdlp **out);

@@ -15,9 +15,9 @@ namespace sov {
 
 // LDS of a phase: the products of one tile, two vectors of T (the CHECK
 // phase) or one vector of doubles (scaling phases, FP64 check passes)
-template <class T, int TNNZ, int BLOCK>
+template <class T, int TNNZ>
 constexpr int lds_doubles() {
-  return kTileMode == 1 ? 2 * (BLOCK / kWave) : (sizeof(T) == 8 ? 2 * TNNZ : TNNZ);
+  return sizeof(T) == 8 ? 2 * TNNZ : TNNZ;
 }
 
 constexpr long long kNoStop = 0x7fffffffffffffffLL;
@@ -104,7 +104,7 @@ template <class T, int BLOCK, int TNNZ>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8)))
     pdlp_run_kernel(const LpDev<T>* __restrict__ lps, const StepCtl* __restrict__ ctl,
                     unsigned* done, int max_phases) {
-  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ, BLOCK>()];
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   __shared__ LpState ss;
   __shared__ Params sp;                  // the parameters, read once per launch
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
 // W workgroups share one LP's phase; each writes its partial sums.
 template <class T, int BLOCK, int TNNZ>
 __global__ void __launch_bounds__(BLOCK) pdlp_phase_kernel(const LpDev<T>* __restrict__ lp) {
-  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ, BLOCK>()];
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   const LpDev<T>& L = *lp;
   const LpState s = *L.st;
@@ -217,7 +217,7 @@ namespace sov {
 template <class T, int BLOCK, int TNNZ>
 __global__ void __launch_bounds__(BLOCK)
     pdlp_slot_phase_kernel(const LpDev<T>* __restrict__ lps, int team) {
-  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ, BLOCK>()];
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   const int slot = (int)blockIdx.x / team, rank = (int)blockIdx.x % team;
   const LpDev<T>& L = lps[slot];

@@ -130,101 +130,6 @@ constexpr int kRowsPerLane = 4;
 // under the run kernel's 128-VGPR cap and is slower.)
 constexpr int kChunk = 4;
 
-// How the short-row tiles of the tile pass read their nonzeros.  Staged (0):
-// a tile's products are loaded with fully coalesced loads into LDS, then,
-// after a workgroup barrier, each row is summed from LDS.  Direct (1): each
-// lane loads the nonzeros of its own rows and sums them as they arrive, with
-// no LDS and no barrier; the additions of every row happen in the same order
-// as in the staged mode, so both modes give the same bits.
-#ifndef SOVLP_TILE_MODE
-#define SOVLP_TILE_MODE 0
-#endif
-constexpr int kTileMode = SOVLP_TILE_MODE;
-
-// s (op)= the products of nonzeros q = a, a + step, a + 2 step, ... < b, in
-// that order.  The loads of kChunk nonzeros are issued before their
-// additions, and the next chunk's column indices load with this chunk's
-// products (a < b required).
-template <int NV, bool MAXOP, class T, class Prod>
-__device__ __forceinline__ void sum_direct(const int32_t* __restrict__ idx, int a, int b,
-                                           int step, Prod& prod, T (&s)[NV]) {
-  int jj[kChunk];
-#pragma unroll
-  for (int c = 0; c < kChunk; ++c) {
-    const int q = a + c * step;
-    jj[c] = idx[q < b ? q : b - 1];
-  }
-  for (int q0 = a; q0 < b; q0 += kChunk * step) {
-    const int qn = q0 + kChunk * step;
-    int jn[kChunk];
-    const bool next = qn < b;
-    if (next) {
-#pragma unroll
-      for (int c = 0; c < kChunk; ++c) {
-        const int q = qn + c * step;
-        jn[c] = idx[q < b ? q : b - 1];
-      }
-    }
-    T v[kChunk][NV];
-#pragma unroll
-    for (int c = 0; c < kChunk; ++c) {
-      const int q = q0 + c * step;
-      prod(q < b ? q : b - 1, jj[c], v[c]);
-    }
-#pragma unroll
-    for (int c = 0; c < kChunk; ++c) {
-      if (q0 + c * step < b) {
-#pragma unroll
-        for (int u = 0; u < NV; ++u) s[u] = MAXOP ? tmax(s[u], v[c][u]) : s[u] + v[c][u];
-      }
-    }
-    if (next) {
-#pragma unroll
-      for (int c = 0; c < kChunk; ++c) jj[c] = jn[c];
-    }
-  }
-}
-
-// The rows tid + k BLOCK (k < kRowsPerLane) of a short-row tile, summed by
-// one lane each, the loads of all of them in flight together: RCH nonzeros
-// of every row per step.  a[k], e[k]: absolute extents (e[k] = a[k] when the
-// row is absent); pz a valid nonzero index for masked lanes.
-template <int NV, bool MAXOP, int RCH, class T, class Prod>
-__device__ __forceinline__ void sum_rows_direct(const int32_t* __restrict__ idx,
-                                                const int (&a)[kRowsPerLane],
-                                                const int (&e)[kRowsPerLane], int pz, Prod& prod,
-                                                T (&s)[kRowsPerLane][NV]) {
-  int len = 0;
-#pragma unroll
-  for (int k = 0; k < kRowsPerLane; ++k) len = e[k] - a[k] > len ? e[k] - a[k] : len;
-  for (int o = 0; o < len; o += RCH) {
-    int jj[kRowsPerLane][RCH];
-    int qq[kRowsPerLane][RCH];
-#pragma unroll
-    for (int k = 0; k < kRowsPerLane; ++k)
-#pragma unroll
-      for (int c = 0; c < RCH; ++c) {
-        const int q = a[k] + o + c;
-        qq[k][c] = q < e[k] ? q : pz;
-        jj[k][c] = idx[qq[k][c]];
-      }
-    T v[kRowsPerLane][RCH][NV];
-#pragma unroll
-    for (int k = 0; k < kRowsPerLane; ++k)
-#pragma unroll
-      for (int c = 0; c < RCH; ++c) prod(qq[k][c], jj[k][c], v[k][c]);
-#pragma unroll
-    for (int k = 0; k < kRowsPerLane; ++k)
-#pragma unroll
-      for (int c = 0; c < RCH; ++c)
-        if (a[k] + o + c < e[k]) {
-#pragma unroll
-          for (int u = 0; u < NV; ++u)
-            s[k][u] = MAXOP ? tmax(s[k][u], v[k][c][u]) : s[k][u] + v[k][c][u];
-        }
-  }
-}
-
 // Stage the products of nonzeros [p0, p0 + nz) of a tile (nz <= TNNZ) into
 // LDS slot q = nonzero - p0.  Lanes past the end load the tile's last nonzero
 // (always valid) and discard it.
@@ -312,13 +217,25 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
       T s[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k) s[k] = T(0);
-      // lane order q = p0 + tid, + BLOCK, ...; kChunk of them in flight at once
-      for (int base = p0; base < p1; base += kChunk * BLOCK) {
-        int jj[kChunk];
+      // lane order q = p0 + tid, + BLOCK, ...; kChunk of them in flight at
+      // once, and the next chunk's column indices load with this chunk's
+      // products (one memory latency per chunk instead of two)
+      int jj[kChunk];
 #pragma unroll
-        for (int c = 0; c < kChunk; ++c) {
-          const int q = base + (int)threadIdx.x + c * BLOCK;
-          jj[c] = idx[q < p1 ? q : p1 - 1];
+      for (int c = 0; c < kChunk; ++c) {
+        const int q = p0 + (int)threadIdx.x + c * BLOCK;
+        jj[c] = idx[q < p1 ? q : p1 - 1];
+      }
+      for (int base = p0; base < p1; base += kChunk * BLOCK) {
+        const int nb = base + kChunk * BLOCK;
+        const bool next = nb < p1;           // uniform: a scalar branch
+        int jn[kChunk];
+        if (next) {
+#pragma unroll
+          for (int c = 0; c < kChunk; ++c) {
+            const int q = nb + (int)threadIdx.x + c * BLOCK;
+            jn[c] = idx[q < p1 ? q : p1 - 1];
+          }
         }
         T v[kChunk][NV];
 #pragma unroll
@@ -334,70 +251,15 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
             for (int k = 0; k < NV; ++k) s[k] = MAXOP ? tmax(s[k], v[c][k]) : s[k] + v[c][k];
           }
         }
+        if (next) {
+#pragma unroll
+          for (int c = 0; c < kChunk; ++c) jj[c] = jn[c];
+        }
       }
       if (more) np1 = uniform(ptr[nr1]);
       block_combine<BLOCK, NV, MAXOP, T>(s, lds);
       if (threadIdx.x == 0) epi(r0, s, pr);
       __syncthreads();
-    } else if (kTileMode == 1 && rows > BLOCK) {
-      // many short rows, direct: one lane per row (rows tid + k BLOCK), the
-      // loads of the lane's rows in flight together, no LDS
-      int a[kRowsPerLane], e[kRowsPerLane];
-      decltype(pre(0)) pr[kRowsPerLane];
-#pragma unroll
-      for (int k = 0; k < kRowsPerLane; ++k) {
-        // rows past the tile load its last row (valid), are empty and not used
-        const int row = (int)threadIdx.x + k * BLOCK;
-        const int rc = r0 + (row < rows ? row : rows - 1);
-        a[k] = ptr[rc];
-        e[k] = row < rows ? ptr[rc + 1] : a[k];
-        pr[k] = pre(rc);
-      }
-      if (more) np1 = uniform(ptr[nr1]);
-      T s[kRowsPerLane][NV];
-#pragma unroll
-      for (int k = 0; k < kRowsPerLane; ++k)
-#pragma unroll
-        for (int u = 0; u < NV; ++u) s[k][u] = T(0);
-      if (nz > 0) sum_rows_direct<NV, MAXOP, (NV > 1 ? 1 : 2), T>(idx, a, e, p0, prod, s);
-#pragma unroll
-      for (int k = 0; k < kRowsPerLane; ++k) {
-        const int row = (int)threadIdx.x + k * BLOCK;
-        if (row < rows) epi(r0 + row, s[k], pr[k]);
-      }
-      // rows past kRowsPerLane * BLOCK (a plan with a larger row cap)
-      for (int row = (int)threadIdx.x + kRowsPerLane * BLOCK; row < rows; row += BLOCK) {
-        const int ra = ptr[r0 + row], rb = ptr[r0 + row + 1];
-        const auto pq = pre(r0 + row);
-        T sq[NV];
-#pragma unroll
-        for (int u = 0; u < NV; ++u) sq[u] = T(0);
-        if (ra < rb) sum_direct<NV, MAXOP, T>(idx, ra, rb, 1, prod, sq);
-        epi(r0 + row, sq, pq);
-      }
-    } else if (kTileMode == 1) {
-      // rows <= BLOCK, direct: tpr lanes per row, lane ln adds the row's
-      // nonzeros ln, ln + tpr, ... as the staged mode adds them from LDS
-      int tpr = BLOCK / (rows > 0 ? rows : 1);
-      tpr = tpr >= 64 ? 64 : (tpr >= 32 ? 32 : (tpr >= 16 ? 16 : (tpr >= 8 ? 8 : (tpr >= 4 ? 4 : (tpr >= 2 ? 2 : 1)))));
-      const int grp = (int)threadIdx.x / tpr, ln = (int)threadIdx.x & (tpr - 1);
-      const bool active = grp < rows;
-      const int rc = r0 + (active ? grp : rows - 1);
-      const int a = ptr[rc], b = ptr[rc + 1];
-      const auto pr = pre(rc);
-      if (more) np1 = uniform(ptr[nr1]);
-      T s[NV];
-#pragma unroll
-      for (int k = 0; k < NV; ++k) s[k] = T(0);
-      if (active && a + ln < b) sum_direct<NV, MAXOP, T>(idx, a + ln, b, tpr, prod, s);
-      for (int o = tpr >> 1; o > 0; o >>= 1) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-          T u = __shfl_xor(s[k], o);
-          s[k] = MAXOP ? tmax(s[k], u) : s[k] + u;
-        }
-      }
-      if (active && ln == 0) epi(r0 + grp, s, pr);
     } else if (rows > BLOCK) {
       // many short rows: one lane per row, up to kRowsPerLane rows per lane,
       // every row's extent and epilogue operands loaded before the products
@@ -480,13 +342,6 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
     r1 = nr1;
     p1 = np1;
   }
-}
-
-// LDS elements of type T a tile pass with NV products needs: the staged
-// tiles (TNNZ per product), or only the long rows' workgroup sums (direct).
-template <int BLOCK, int TNNZ, int NV>
-constexpr int tile_lds_elems() {
-  return kTileMode == 1 ? NV * (BLOCK / kWave) : NV * TNNZ;
 }
 
 // The tile pass without epilogue prefetch: epi(row, s[NV]).

@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(BLOCK)
                       const T* __restrict__ val, const int32_t* __restrict__ tiles,
                       const int32_t* __restrict__ part, const T* __restrict__ x,
                       T* __restrict__ y) {
-  __shared__ __align__(16) T lds[tile_lds_elems<BLOCK, TNNZ, 1>()];
+  __shared__ __align__(16) T lds[TNNZ];
   const int t0 = part[blockIdx.x], t1 = part[blockIdx.x + 1];
   auto prod = [&](int q, int j, T (&v)[1]) { v[0] = val[q] * x[j]; };
   auto epi = [&](int r, const T (&s)[1]) { y[r] = s[0]; };
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(BLOCK)
                       const int32_t* __restrict__ tiles, const int32_t* __restrict__ tile_off,
                       const int32_t* __restrict__ parts, int groups, const T* __restrict__ x,
                       T* __restrict__ partial) {
-  __shared__ __align__(16) T lds[tile_lds_elems<BLOCK, TNNZ, 1>()];
+  __shared__ __align__(16) T lds[TNNZ];
   const int p = (int)blockIdx.x % panels, g = (int)blockIdx.x / panels;
   const int32_t* pp = parts + (size_t)p * (groups + 1);
   const int t0 = pp[g], t1 = pp[g + 1];
@@ -276,6 +276,23 @@ static int list_long_lines(int lines, const int64_t* ptr, int* counts, int* tota
   return 0;
 }
 
+// The residual's launches over a scratch of sovlp_residual_workspace_bytes:
+// the long rows' count, their list, then the listing's per-chunk counts.
+// Every word the kernels read is written first, so the scratch needs no
+// clearing.
+static int residual_launch(int rows, const int64_t* rp, const int32_t* ri, const double* rx,
+                           const double* x, const double* b, double* out, int* cnt,
+                           hipStream_t st) {
+  const int groups = std::min(kCertMaxGroups * 4, (rows + kCertBlock - 1) / kCertBlock);
+  if (int rc = list_long_lines(rows, rp, cnt + 4 + rows, cnt, cnt + 4, st)) return rc;
+  residual_kernel<<<groups, kCertBlock, 0, st>>>(rows, rp, ri, rx, x, b, out);
+  HIPCHK(hipGetLastError());
+  long_lines_kernel<true><<<kCertMaxGroups, kCertBlock, 0, st>>>(cnt, cnt + 4, rp, ri, rx, x, b,
+                                                                   1.0, out);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int sovlp_residual_f64(int rows, const int64_t* rp, const int32_t* ri, const double* rx,
                        const double* x, const double* b, double* out, void* workspace,
                        int64_t workspace_bytes, sovlp_stream_t stream) {
@@ -286,18 +303,9 @@ int sovlp_residual_f64(int rows, const int64_t* rp, const int32_t* ri, const dou
   sovlp_residual_workspace_bytes(rows, &need);
   if (workspace_bytes < need) return fail(1, "workspace too small (sovlp_residual_workspace_bytes)");
   if (reinterpret_cast<uintptr_t>(workspace) % 4) return fail(1, "workspace not 4-byte aligned");
-  // the count of long rows, their list, then the listing's per-chunk counts
-  int* cnt = static_cast<int*>(workspace);
-  hipStream_t st = S(stream);
-  const int groups = std::min(kCertMaxGroups * 4, (rows + kCertBlock - 1) / kCertBlock);
-  if (int rc = list_long_lines(rows, rp, cnt + 4 + rows, cnt, cnt + 4, st)) return rc;
-  residual_kernel<<<groups, kCertBlock, 0, st>>>(rows, rp, ri, rx, x, b, out);
-  HIPCHK(hipGetLastError());
-  long_lines_kernel<true><<<kCertMaxGroups, kCertBlock, 0, st>>>(cnt, cnt + 4, rp, ri, rx, x, b,
-                                                                   1.0, out);
-  HIPCHK(hipGetLastError());
-  return 0;
+  return residual_launch(rows, rp, ri, rx, x, b, out, static_cast<int*>(workspace), S(stream));
 }
+
 
 static int cert_groups(int len) {
   return std::max(1, std::min(kCertMaxGroups, (len + kCertBlock - 1) / kCertBlock));

@@ -11,7 +11,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsovlp.so")
+# SOVLP_LIBRARY names another build of the same sources (e.g. one compiled
+# with other -D options, for comparing them); the in-tree build otherwise
+LIB_PATH = os.environ.get("SOVLP_LIBRARY") or os.path.join(HERE, "libsovlp.so")
 
 _lib = None
 
