@@ -66,6 +66,20 @@ int sovlp_residual_f64(int rows, const int64_t *rp, const int32_t *ri, const dou
                        const double *x, const double *b, double *out, void *workspace,
                        int64_t workspace_bytes, sovlp_stream_t stream);
 
+/* sovlp_residual_f64 without a workspace argument and without allocating:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its scratch is a device array of the library holding
+ * the workspace of up to sovlp_residual_noalloc_max_rows() rows (1048576, 4
+ * MiB), one copy per GPU, kept until the process ends; larger inputs are
+ * refused (code 1).  Every call on a GPU shares that GPU's copy, so calls
+ * that overlap on one GPU race; calls ordered on one stream do not.  Same
+ * results as sovlp_residual_f64, bit for bit. */
+int sovlp_residual_noalloc_max_rows(void);
+int sovlp_residual_f64_noalloc(int rows, const int64_t *rp, const int32_t *ri, const double *rx,
+                               const double *x, const double *b, double *out,
+                               sovlp_stream_t stream);
+
 /* The certificate of a point x and dual vector y (textbook signs, d = c - A^T y)
  * on an LP in the model's own sense:
  *   bound          certified_bound (src/sovopt/mip/safebound.py:252-307):

@@ -8,6 +8,8 @@
 //   reduce_kernel        Backend surface dot / nrm2 / nrm_inf, fixed order
 //   spmv_tiled_kernel    the engines' CSR-Adaptive tile pass as a bare SpMV
 #include "common.hpp"
+
+#include <mutex>
 #include "certify_kernels.hpp"
 #include "bnr_kernels.hpp"
 
@@ -306,6 +308,54 @@ int sovlp_residual_f64(int rows, const int64_t* rp, const int32_t* ri, const dou
   return residual_launch(rows, rp, ri, rx, x, b, out, static_cast<int*>(workspace), S(stream));
 }
 
+}  // extern "C"
+
+// Scratch of sovlp_residual_f64_noalloc: the residual's workspace for up to
+// kNoAllocRows rows (4 MiB), one copy on each GPU for the life of the process.
+// Its name is a host-side stand-in: host code uses only the device address
+// that hipGetSymbolAddress returns for the current device.
+constexpr int kNoAllocRows = 1 << 20;
+static __device__ __align__(256) int32_t scratch[4 + kNoAllocRows + kCertMaxGroups];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<int*> g_scratch;     // per device: the address of its copy of `scratch`
+
+// The current device's copy of `scratch`, looked up once per device.
+int scratch_on_current_device(int** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0) return fail(2, "no current HIP device");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1, nullptr);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(2, "hipGetSymbolAddress returned no address for the scratch");
+    g_scratch[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int sovlp_residual_noalloc_max_rows(void) { return kNoAllocRows; }
+
+int sovlp_residual_f64_noalloc(int rows, const int64_t* rp, const int32_t* ri, const double* rx,
+                               const double* x, const double* b, double* out,
+                               sovlp_stream_t stream) {
+  if (rows < 0) return fail(1, "negative row count");
+  if (rows > kNoAllocRows)
+    return fail(1, "sovlp_residual_f64_noalloc takes at most 1048576 rows "
+                   "(use sovlp_residual_f64 with a workspace)");
+  if (rows == 0) return 0;
+  if (!rp || !ri || !rx || !x || !b || !out) return fail(1, "null argument");
+  int* cnt = nullptr;
+  if (int rc = scratch_on_current_device(&cnt)) return rc;
+  return residual_launch(rows, rp, ri, rx, x, b, out, cnt, S(stream));
+}
 
 static int cert_groups(int len) {
   return std::max(1, std::min(kCertMaxGroups, (len + kCertBlock - 1) / kCertBlock));

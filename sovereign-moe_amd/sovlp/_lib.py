@@ -142,6 +142,8 @@ _SIGS = {
     "sovlp_bnr_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(I64)]),
     "sovlp_bnr_solve_batch": (C.c_int, [C.POINTER(sovlp_bnr_batch), P, I64, P]),
     "sovlp_residual_f64": (C.c_int, [C.c_int, P, P, P, P, P, P, P, I64, P]),
+    "sovlp_residual_noalloc_max_rows": (C.c_int, []),
+    "sovlp_residual_f64_noalloc": (C.c_int, [C.c_int, P, P, P, P, P, P, P]),
     "sovlp_certify_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.POINTER(I64)]),
     "sovlp_certify_f64": (C.c_int, [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P, F64,
                                     C.c_int, P, P, F64, P, I64, P, P,
