@@ -55,6 +55,17 @@ int sovlp_spmv_csr_f64(int rows, const int64_t *rp, const int32_t *ri, const dou
 int sovlp_spmv_csr_f32(int rows, const int64_t *rp, const int32_t *ri, const float *rx,
                        const float *x, float *y, int lanes_per_row, sovlp_stream_t stream);
 
+/* out = A X with X dense (cols, k) and out (rows, k), row-major with the
+ * batch index fastest (A in CSR, int64 pointers): one lane per (row, column)
+ * walking the row's nonzeros in order, products and sums unfused, so the
+ * results are the reference's CPU _csr_spmm (src/sovopt/core/sparse.py:72-83)
+ * bit for bit.  Replaces Backend.spmm, backend.py:386-400 (kernels
+ * csr_spmm_warp_*, backend.py:158-200). */
+int sovlp_spmm_csr_f64(int rows, const int64_t *rp, const int32_t *ri, const double *rx,
+                       const double *X, double *out, int k, sovlp_stream_t stream);
+int sovlp_spmm_csr_f32(int rows, const int64_t *rp, const int32_t *ri, const float *rx,
+                       const float *X, float *out, int k, sovlp_stream_t stream);
+
 /* out = b - A x to about double-double accuracy (A in CSR, int64 pointers):
  * each row's dot product with Dekker's two_product and Knuth's two_sum in
  * CSR order, bit for bit the reference's compensated_residual

@@ -13,7 +13,9 @@ the device's fixed order -- blocks of ``256 / KP`` rows (KP: K rounded up to a
 power of two), then the blocks over 256 lanes in a fixed order -- where the reference's NumPy sums
 run down the rows.  The interaction sum can cancel heavily, so any other
 order moves the step sizes visibly; with the same order the whole batch is
-the device's bit for bit.
+the device's bit for bit.  ``order="rows"`` sums as the reference does; run
+that way the oracle reproduces the reference's own batches
+(``tests/golden/reference_bnr.json``, written by running ``BNREngine``).
 """
 
 from __future__ import annotations
@@ -65,8 +67,20 @@ def blocked_sum(V, rb: int):
     return total
 
 
-def safe_dual_bound_batch(prob, L, U, Y):
-    """(K,) bounds from the (m, K) duals; -inf where a term is vacuous."""
+def row_sum(V):
+    """Per-column sum of V (rows, K) as the reference takes it: NumPy's
+    ``V.sum(axis=0)`` of a C-ordered array (down the rows for K > 1; NumPy's
+    pairwise order when K == 1 makes the rows contiguous)."""
+    return np.ascontiguousarray(V).sum(axis=0)
+
+
+def _sum(V, rb: int, order: str):
+    return blocked_sum(V, rb) if order == "device" else row_sum(V)
+
+
+def safe_dual_bound_batch(prob, L, U, Y, order: str = "device"):
+    """(K,) bounds from the (m, K) duals; -inf where a term is vacuous.
+    ``order``: "device" sums in the device's order, "rows" in the reference's."""
     A = prob.A
     aty = spmm(A.cp, A.ci, A.cx, Y, A.n)
     d = prob.c[:, None] - aty
@@ -81,15 +95,17 @@ def safe_dual_bound_batch(prob, L, U, Y):
     col_c = np.where(col_bad, 0.0, d * np.where(col_bad, 0.0, col_pick))
     vac = row_bad.any(axis=0) | col_bad.any(axis=0)
     rb = block_rows(Y.shape[1])
-    s = blocked_sum(row_c, rb) if row_c.shape[0] else np.zeros(Y.shape[1])
-    t = blocked_sum(col_c, rb)
+    s = _sum(row_c, rb, order) if row_c.shape[0] else np.zeros(Y.shape[1])
+    t = _sum(col_c, rb, order)
     return np.where(vac, -np.inf, s + t)
 
 
 def solve_batch(prob, L, U, iters=60, warm=None, eta=None, omega=None, growth=0.3, damp=0.6,
-                dual_sign=1.0):
+                dual_sign=1.0, order: str = "device"):
     """One batch visit: returns (bounds scaled, X, Y, eta, empty).  The bound
-    reads ``dual_sign * Y`` (1: as the reference; -1: textbook signs)."""
+    reads ``dual_sign * Y`` (1: as the reference; -1: textbook signs).
+    ``order`` of the per-node sums: "device" (the GPU's fixed order) or "rows"
+    (down the rows, the reference's NumPy order)."""
     A = prob.A
     m, n, K = A.m, A.n, L.shape[1]
     L = np.asarray(L, dtype=np.float64)
@@ -116,9 +132,9 @@ def solve_batch(prob, L, U, iters=60, warm=None, eta=None, omega=None, growth=0.
         dY = Yn - Y
         AtdY = spmm(A.cp, A.ci, A.cx, dY, n)
         rb = block_rows(K)
-        inter = blocked_sum(dY * AdX, rb) if m else np.zeros(K)
-        dyy = blocked_sum(dY * dY, rb) if m else np.zeros(K)
-        dxx = blocked_sum(dX * dX, rb)
+        inter = _sum(dY * AdX, rb, order) if m else np.zeros(K)
+        dyy = _sum(dY * dY, rb, order) if m else np.zeros(K)
+        dxx = _sum(dX * dX, rb, order)
         interaction = np.abs(inter)
         movement = 0.5 * (omega * dxx + dyy / omega)
         floor = np.maximum(movement, 1.0) / QUOTIENT_MAX
@@ -133,5 +149,5 @@ def solve_batch(prob, L, U, iters=60, warm=None, eta=None, omega=None, growth=0.
         t = attempts + 1
         eta = np.minimum((1.0 - t ** -growth) * limit, (1.0 + t ** -damp) * eta)
         eta = np.maximum(np.where(np.isfinite(eta), eta, 1.0), 1e-14)
-    bounds = safe_dual_bound_batch(prob, L, U, dual_sign * Y)
+    bounds = safe_dual_bound_batch(prob, L, U, dual_sign * Y, order)
     return bounds, X, Y, eta, empty

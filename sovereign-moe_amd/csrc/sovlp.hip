@@ -93,6 +93,53 @@ int launch_spmv_vector(int rows, const int64_t* rp, const int32_t* ri, const T* 
   return 0;
 }
 
+// out = A X with X dense (cols, k) and out (rows, k), batch index fastest:
+// thread g takes row g / k and column g % k and walks the row's nonzeros in
+// CSR order from 0, multiplying and adding separately (the reference's
+// _csr_spmm, core/sparse.py:72-83), so results equal its CPU ones bit for bit.
+template <class T>
+__global__ void __launch_bounds__(256)
+    spmm_csr_kernel(int rows, int k, const int64_t* __restrict__ rp,
+                    const int32_t* __restrict__ ri, const T* __restrict__ rx,
+                    const T* __restrict__ X, T* __restrict__ out) {
+#pragma clang fp contract(off)
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= (long long)rows * k) return;
+  const long long row = g / k;
+  const int t = (int)(g % k);
+  T acc = T(0);
+  const int64_t p1 = rp[row + 1];
+  int64_t p = rp[row];
+  for (; p + 4 <= p1; p += 4) {          // four nonzeros' loads, then the sums in order
+    int jj[4];
+    T a[4], v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      jj[u] = ri[p + u];
+      a[u] = rx[p + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = X[(long long)jj[u] * k + t];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += a[u] * v[u];
+  }
+  for (; p < p1; ++p) acc += rx[p] * X[(long long)ri[p] * k + t];
+  out[g] = acc;
+}
+
+template <class T>
+int launch_spmm(int rows, const int64_t* rp, const int32_t* ri, const T* rx, const T* X, T* out,
+                int k, hipStream_t st) {
+  if (rows < 0 || k < 0) return fail(1, "rows and k must be >= 0");
+  if (rows == 0 || k == 0) return 0;
+  const long long total = (long long)rows * k;
+  if (total > (1LL << 40)) return fail(1, "rows x k too large");
+  spmm_csr_kernel<T><<<(unsigned)((total + 255) / 256), 256, 0, st>>>(rows, k, rp, ri, rx, X,
+                                                                      out);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 __global__ void primal_step_kernel(double* __restrict__ xnew, const double* __restrict__ x,
                                    const double* __restrict__ c, const double* __restrict__ aty,
                                    const double* __restrict__ lo, const double* __restrict__ hi,
@@ -253,6 +300,15 @@ int sovlp_spmv_csr_f64(int rows, const int64_t* rp, const int32_t* ri, const dou
 int sovlp_spmv_csr_f32(int rows, const int64_t* rp, const int32_t* ri, const float* rx,
                        const float* x, float* y, int lanes_per_row, sovlp_stream_t stream) {
   return launch_spmv_vector<float>(rows, rp, ri, rx, x, y, lanes_per_row, S(stream));
+}
+
+int sovlp_spmm_csr_f64(int rows, const int64_t* rp, const int32_t* ri, const double* rx,
+                       const double* X, double* out, int k, sovlp_stream_t stream) {
+  return launch_spmm<double>(rows, rp, ri, rx, X, out, k, S(stream));
+}
+int sovlp_spmm_csr_f32(int rows, const int64_t* rp, const int32_t* ri, const float* rx,
+                       const float* X, float* out, int k, sovlp_stream_t stream) {
+  return launch_spmm<float>(rows, rp, ri, rx, X, out, k, S(stream));
 }
 
 // ---------------------------------------------------------------- certificate

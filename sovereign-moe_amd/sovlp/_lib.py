@@ -141,6 +141,8 @@ _SIGS = {
     "sovlp_residual_workspace_bytes": (C.c_int, [C.c_int, C.POINTER(I64)]),
     "sovlp_bnr_workspace_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(I64)]),
     "sovlp_bnr_solve_batch": (C.c_int, [C.POINTER(sovlp_bnr_batch), P, I64, P]),
+    "sovlp_spmm_csr_f64": (C.c_int, [C.c_int, P, P, P, P, P, C.c_int, P]),
+    "sovlp_spmm_csr_f32": (C.c_int, [C.c_int, P, P, P, P, P, C.c_int, P]),
     "sovlp_residual_f64": (C.c_int, [C.c_int, P, P, P, P, P, P, P, I64, P]),
     "sovlp_residual_noalloc_max_rows": (C.c_int, []),
     "sovlp_residual_f64_noalloc": (C.c_int, [C.c_int, P, P, P, P, P, P, P]),

@@ -1,7 +1,7 @@
 """Device backend: PyTorch-ROCm buffers plus the hand-written gfx950 kernels.
 
 Mirrors the reference ``Backend`` (``src/sovopt/core/backend.py:317-551``):
-``spmv``, ``primal_step``, ``dual_step``, ``pdhg_primal``, ``pdhg_dual``,
+``spmv``, ``spmm``, ``primal_step``, ``dual_step``, ``pdhg_primal``, ``pdhg_dual``,
 ``pdhg_accum``, ``dot``, ``nrm2``, ``nrm_inf``, ``to_device``, ``to_host``,
 ``index``, ``pointer``, ``zeros``, ``empty``, ``sync``, and the module-level
 ``get_backend``, ``gpu_available``, ``gpu_selftest`` and ``GPU_ERROR``.
@@ -130,6 +130,13 @@ class Backend:
         lanes = lanes_per_row or self.lanes_for(m, rx.numel())
         fn = "sovlp_spmv_csr_f32" if rx.dtype == _torch.float32 else "sovlp_spmv_csr_f64"
         _lib.call(fn, int(m), _p(rp), _p(ri), _p(rx), _p(x), _p(out), lanes, _stream())
+        return out
+
+    def spmm(self, rp, ri, rx, X, out, m, k_batch):
+        """``out = A X`` with ``X`` dense ``(n, k)``, batch index fastest
+        (``backend.py:386-400``): the reference's CPU ``_csr_spmm`` bit for bit."""
+        fn = "sovlp_spmm_csr_f32" if rx.dtype == _torch.float32 else "sovlp_spmm_csr_f64"
+        _lib.call(fn, int(m), _p(rp), _p(ri), _p(rx), _p(X), _p(out), int(k_batch), _stream())
         return out
 
     # -- fused steps

@@ -1,10 +1,17 @@
 """The batched-node-relaxation oracle (oracle/bnr_oracle.py), pinned on the CPU.
 
-Pins: at the exact optimal duals of a known-optimum LP the batched bound is
-the optimum (closed form); for any iterate it is below each node's optimum as
-HiGHS computes it (scipy); an empty box is flagged; with textbook-signed duals
-the bound converges to the optimum; a one-variable case by hand.
+Pins: the reference's own batches (``tests/golden/reference_bnr.json``, written
+by running its ``BNREngine.solve_batch`` on the CPU: ``make_bnr_vectors.py``),
+reproduced bit for bit with the reference's summation order and to rounding
+with the device's; at the exact optimal duals of a known-optimum LP the
+batched bound is the optimum (closed form); for any iterate it is below each
+node's optimum as HiGHS computes it (scipy); an empty box is flagged; with
+textbook-signed duals the bound converges to the optimum; a one-variable case
+by hand.
 """
+
+import json
+import os
 
 import numpy as np
 import pytest
@@ -40,6 +47,60 @@ def highs_optima(p, L, U):
                     method="highs")
         out.append(r.fun if r.status == 0 else np.inf)
     return np.array(out)
+
+
+REF = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_bnr.json")))
+
+
+def reference_case(c):
+    """(problem, L, U, visits) of a golden case; bounds as floats."""
+    A = SparseMatrix.from_triplets(np.array(c["rows"]), np.array(c["cols"]),
+                                   np.array(c["vals"], float), c["m"], c["n"])
+    p = Problem(A=A, c=np.array(c["c"]), row_lb=np.array(c["rl"]), row_ub=np.array(c["ru"]),
+                col_lb=np.zeros(c["n"]), col_ub=np.full(c["n"], 10.0))
+    visits = []
+    for v in c["visits"]:
+        visits.append(dict(X=np.array(v["X"]), Y=np.array(v["Y"]), eta=np.array(v["eta"]),
+                           omega=np.array(v["omega"]),
+                           bounds=np.array([float(b) for b in v["bounds"]]),
+                           empty=np.array(v["empty"])))
+    return p, np.array(c["L"]), np.array(c["U"]), visits
+
+
+@pytest.mark.parametrize("case", REF["cases"], ids=[c["name"] for c in REF["cases"]])
+def test_reference_order_reproduces_the_reference_bit_for_bit(case):
+    """Cold then warm visits (the second starting from the first's X, Y and step
+    sizes, as the engine carries them): X, Y, step sizes and flags bit for bit;
+    the bounds too, with an empty node's bound reported as +inf as the engine
+    does (bnr.py:285-286)."""
+    p, L, U, visits = reference_case(case)
+    warm = eta = omega = None
+    for v in visits:
+        b, X, Y, e, empty = bo.solve_batch(p, L, U, iters=case["iters"], warm=warm, eta=eta,
+                                           omega=omega, order="rows")
+        assert np.array_equal(X, v["X"]) and np.array_equal(Y, v["Y"])
+        assert np.array_equal(e, v["eta"])
+        assert np.array_equal(empty, v["empty"])
+        assert np.array_equal(np.where(empty, np.inf, b), v["bounds"])
+        warm, eta, omega = (v["X"], v["Y"]), v["eta"], v["omega"]
+
+
+@pytest.mark.parametrize("case", REF["cases"], ids=[c["name"] for c in REF["cases"]])
+def test_device_order_agrees_with_the_reference_to_rounding(case):
+    """The device's summation order (what the GPU is compared with) moves the
+    reference's batches by rounding only."""
+    p, L, U, visits = reference_case(case)
+    warm = eta = omega = None
+    for v in visits:
+        b, X, Y, e, empty = bo.solve_batch(p, L, U, iters=case["iters"], warm=warm, eta=eta,
+                                           omega=omega, order="device")
+        for got, want in ((X, v["X"]), (Y, v["Y"]), (e, v["eta"])):
+            assert np.all(np.abs(got - want) <= 1e-12 * np.maximum(1.0, np.abs(want)))
+        bb, rb = np.where(empty, np.inf, b), v["bounds"]
+        fin = np.isfinite(rb)
+        assert np.array_equal(np.isfinite(bb), fin) and np.array_equal(bb[~fin], rb[~fin])
+        assert np.all(np.abs(bb[fin] - rb[fin]) <= 1e-12 * np.maximum(1.0, np.abs(rb[fin])))
+        warm, eta, omega = (v["X"], v["Y"]), v["eta"], v["omega"]
 
 
 def test_the_bound_at_the_optimal_duals_is_the_optimum():

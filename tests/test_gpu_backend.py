@@ -251,3 +251,31 @@ def test_tiled_spmv_with_columns_in_locality_order_is_bitwise_the_same(dtype):
     np.testing.assert_allclose(tb[new], t, rtol=1e-12 if f64 else 1e-4, atol=1e-12 if f64 else 2e-3)
     np.testing.assert_allclose(y, A.matvec(x.astype(np.float64)),
                                rtol=1e-10 if f64 else 1e-3, atol=1e-10 if f64 else 1e-3)
+
+
+@pytest.mark.parametrize("k", [1, 3, 64, 300])
+def test_spmm_is_the_reference_cpu_product_bit_for_bit(k):
+    """``Backend.spmm`` (backend.py:386-400): out = A X, batch index fastest,
+    with empty rows and rows of 1..9 nonzeros; FP64 against the oracle's CSR-order
+    product (the reference's _csr_spmm), FP32 against the same in float32."""
+    from oracle import bnr_oracle
+    bk = get_backend("gpu")
+    A, rng = _random_csr(70, 50, 3, 41 + k)
+    X = rng.standard_normal((A.n, k))
+    out = bk.zeros((A.m, k))
+    bk.spmm(*_to(bk, A), bk.to_device(X), out, A.m, k)
+    ref = bnr_oracle.spmm(A.rp, A.ri, A.rx, X, A.m)
+    assert np.array_equal(bk.to_host(out), ref)
+    b32 = get_backend("gpu", np.float32)
+    X32 = X.astype(np.float32)
+    rx32 = A.rx.astype(np.float32)
+    out32 = b32.zeros((A.m, k))
+    b32.spmm(bk.pointer(A.rp), bk.index(A.ri), b32.to_device(rx32), b32.to_device(X32), out32,
+             A.m, k)
+    ref32 = np.zeros((A.m, k), np.float32)
+    for i in range(A.m):
+        acc = np.zeros(k, np.float32)
+        for p in range(A.rp[i], A.rp[i + 1]):
+            acc = acc + rx32[p] * X32[A.ri[p]]
+        ref32[i] = acc
+    assert np.array_equal(b32.to_host(out32), ref32)

@@ -15,7 +15,7 @@ from sovlp.core.problem import Problem
 from sovlp.core.sparse import SparseMatrix
 from sovlp.workloads import known_optimum_lp
 
-from test_bnr_oracle import highs_optima, node_boxes
+from test_bnr_oracle import REF, highs_optima, node_boxes, reference_case
 
 pytestmark = pytest.mark.gpu
 
@@ -55,6 +55,29 @@ def test_batches_match_the_oracle(m_eq, K, iters, textbook):
     opt = highs_optima(p, L, U)
     ok = ~empty
     assert np.all(r2.bounds[ok] <= opt[ok] + 1e-9 * np.maximum(1.0, np.abs(opt[ok])))
+
+
+@pytest.mark.parametrize("case", REF["cases"], ids=[c["name"] for c in REF["cases"]])
+def test_batches_match_the_reference_run(case):
+    """The reference's own batches (its BNREngine run on the CPU, cold then warm
+    with the step sizes the engine keeps): the device's X, Y, step sizes and
+    bounds to 1e-12 relative (the per-node sums run in another order), the
+    empty flags exactly."""
+    from sovlp.mip.bnr import BNRConfig, BNREngine
+    print(f"[gpu] bnr vs reference {case['name']}", flush=True)
+    p, L, U, visits = reference_case(case)
+    K = case["K"]
+    eng = BNREngine(p, config=BNRConfig(batch=K, iters=case["iters"]))
+    warm = None
+    for v in visits:
+        r = eng.solve_batch(L, U, warm=warm)
+        X, Y = eng.iterates(K)
+        assert np.array_equal(r.infeasible, v["empty"])
+        _close(r.bounds, v["bounds"], 1e-12)
+        _close(X, v["X"], 1e-12)
+        _close(Y, v["Y"], 1e-12)
+        _close(eng.eta[:K], v["eta"], 1e-12)
+        warm = (X, Y)
 
 
 def test_a_model_without_rows():
