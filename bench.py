@@ -284,6 +284,52 @@ def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: in
     return float(d["traffic_bytes_per_launch"])
 
 
+def slowest_lps(eng, sols, lat, problems, k: int = 8) -> list:
+    """The LPs that set the step: index, size, kernel block, iterations,
+    refinement rounds, status, start and latency on the device clock (ms from
+    the step's first LP start)."""
+    import ctypes
+    import numpy as np
+    from sovlp import _lib
+    hz = ctypes.c_double()
+    _lib.call("sovlp_clock_rate", 0, ctypes.byref(hz))
+    infos = eng.infos()
+    t0 = min(int(infos[i].start_tick) for i in range(eng.count))
+    out = []
+    for i in np.argsort(lat)[::-1][:k]:
+        i = int(i)
+        p = problems[i]
+        out.append({"lp": i, "m": p.m, "n": p.n, "nnz": int(p.A.nnz),
+                    "block": int(eng.db.blocks[i]), "iterations": int(sols[i].iterations),
+                    "rounds": len(sols[i].info.refinement), "status": sols[i].status.name,
+                    "start_ms": (int(infos[i].start_tick) - t0) / hz.value * 1e3,
+                    "latency_ms": float(lat[i]) * 1e3})
+    return out
+
+
+def phase_profile(counts, ticks, lat) -> dict:
+    """Device time per phase of the last step, from the run kernel's own clock:
+    for all LPs and for the 1 % slowest (the step's tail), each phase's calls
+    and mean microseconds per call."""
+    import ctypes
+    import numpy as np
+    from sovlp import _lib
+    L = _lib.lib()
+    hz = ctypes.c_double()
+    _lib.call("sovlp_clock_rate", 0, ctypes.byref(hz))
+    names = [L.sovlp_phase_name(k).decode() for k in range(counts.shape[1])]
+    tail = np.argsort(lat)[-max(1, len(lat) // 100):]
+    out = {}
+    for label, rows in (("all", slice(None)), ("slowest_1pct", tail)):
+        c = counts[rows].sum(axis=0)
+        t = ticks[rows].sum(axis=0) / hz.value * 1e6
+        out[label] = {names[k]: {"calls": int(c[k]), "us_per_call": float(t[k] / c[k]),
+                                 "share": float(t[k] / t.sum())}
+                      for k in range(len(names)) if c[k] > 0 and names[k]
+                      and not names[k].endswith("_NEXT")}
+    return out
+
+
 def plain_fp64_rate(problems, tol, sample):
     """LPs/s of the single-LP FP64 engine, one LP at a time, inputs resident."""
     import torch
@@ -614,6 +660,8 @@ def main():
     rel = [abs(s.objective - k.f_star) / max(1.0, abs(k.f_star))
            for s, k in zip(sols, kos) if int(s.status) == 0]
     counts = eng.phase_counts()
+    phase_prof = phase_profile(counts, eng.phase_ticks(), lat)
+    slowest = slowest_lps(eng, sols, lat, problems)
     step_bytes = bytes_of_counts(counts, eng.sizes(), a.precision)
     launches = sum(graphs)            # one run-kernel launch per graph replay
     launch_ms = dev_ms / launches
@@ -655,6 +703,8 @@ def main():
         "mean_iterations": float(np.mean([s.iterations for s in sols])),
         "mean_refinement_rounds": float(np.mean([len(s.info.refinement) for s in sols])),
         "escalated_to_fp64": len(esc),
+        "phase_profile": phase_prof,
+        "slowest_lps": slowest,
         "escalation_s_per_step": esc_s / a.steps,
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -374,6 +374,12 @@ const char *sovlp_phase_name(int slot);
 /* counts[sovlp_phase_slots()] of the engine's LP; synchronises. */
 int sovlp_pdlp_phase_counts(sovlp_pdlp *h, int64_t *counts, sovlp_stream_t stream);
 
+/* ticks[sovlp_phase_slots()]: device clock ticks (sovlp_clock_rate) the LP
+ * spent in each phase, from the end of the previous phase's transition to
+ * the end of its own, as the one-workgroup run kernel measures them (zero
+ * with W workgroups per phase); synchronises.  No reference counterpart. */
+int sovlp_pdlp_phase_ticks(sovlp_pdlp *h, int64_t *ticks, sovlp_stream_t stream);
+
 /* ---- batched PDLP: many independent LPs in one persistent launch ------
  * No reference counterpart: the reference solves one LP per call
  * (pdlp.py:238); its only batched PDHG shares one A across B&B nodes
@@ -385,8 +391,8 @@ int sovlp_pdlp_phase_counts(sovlp_pdlp *h, int64_t *counts, sovlp_stream_t strea
  * Tile plans must use sovlp_batch_geometry's tile size and row count. */
 typedef struct sovlp_batch sovlp_batch;
 
-/* Tile geometry of the batched engine for `precision` and `block` (64, 128
- * or 256 threads per LP): the row cap and nonzero cap of a tile. */
+/* Tile geometry of the batched engine for `precision` and `block` (64, 128,
+ * 256 or 512 threads per LP): the row cap and nonzero cap of a tile. */
 int sovlp_batch_geometry(int precision, int block, int *tile_rows, int *tile_nnz);
 
 /* Workspace for `count` LPs (host array of descriptors with device pointers). */
@@ -395,14 +401,14 @@ int sovlp_batch_workspace_bytes(int count, const sovlp_lp *lps, int precision,
 
 /* Lays out the batch in the workspace and records one replay into a
  * hipGraph: one launch, one workgroup of `block` threads per LP, each running
- * up to 65536 phases of its LP back to back.  `block` is 64, 128 or 256.
+ * up to 65536 phases of its LP back to back.  `block` is 64, 128, 256 or 512.
  * Synchronises. */
 int sovlp_batch_create(int count, const sovlp_lp *lps, const sovlp_params *params,
                        int precision, int block, void *workspace, int64_t workspace_bytes,
                        int64_t history_cap, sovlp_stream_t stream, sovlp_batch **out);
 
 /* As sovlp_batch_create with a workgroup size per LP (blocks[count], each 64,
- * 128 or 256; an autotuner's choice from the LP's own shape).  A replay
+ * 128, 256 or 512; an autotuner's choice from the LP's own shape).  A replay
  * launches the LPs of each size in one launch of their own, one after the
  * other; each LP's tile plans use sovlp_batch_geometry of its own size. */
 int sovlp_batch_create_mixed(int count, const sovlp_lp *lps, const sovlp_params *params,
@@ -427,6 +433,10 @@ int sovlp_batch_refinement(sovlp_batch *h, int index, double **rows);
 
 /* counts[count * sovlp_phase_slots()], LP-major; synchronises. */
 int sovlp_batch_phase_counts(sovlp_batch *h, int64_t *counts, sovlp_stream_t stream);
+
+/* ticks[count * sovlp_phase_slots()], LP-major: device clock ticks each LP
+ * spent in each phase (as sovlp_pdlp_phase_ticks); synchronises. */
+int sovlp_batch_phase_ticks(sovlp_batch *h, int64_t *ticks, sovlp_stream_t stream);
 
 int sovlp_batch_destroy(sovlp_batch *h);
 
@@ -456,7 +466,7 @@ int sovlp_server_lp_bytes(const sovlp_lp *lp, int precision, int64_t history_cap
                           int64_t *bytes);
 
 /* Creates an empty server (every slot free) and records its launch graph.
- * `block` 64, 128 or 256; params->time_limit is each LP's device time budget,
+ * `block` 64, 128, 256 or 512; params->time_limit is each LP's device time budget,
  * counted from its first phase.  Synchronises. */
 int sovlp_server_create(int slots, const sovlp_params *params, int precision, int block,
                         int phases_per_launch, void *workspace, int64_t workspace_bytes,

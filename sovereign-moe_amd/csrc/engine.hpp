@@ -17,6 +17,7 @@ struct sovlp_pdlp {
   unsigned* done;
   double *xo, *yo, *dout, *hist, *rhist;
   long long* pcount;         // [kPhaseSlots]
+  long long* pticks;         // [kPhaseSlots]
   int phases_per_replay;
   long long hist_cap;
   double hz;
@@ -37,6 +38,7 @@ struct sovlp_batch {
   StepCtl* ctl;
   unsigned* done;
   long long* pcount;         // [count][kPhaseSlots]
+  long long* pticks;         // [count][kPhaseSlots]
   std::vector<double*> xo, yo, dout, rhist;
   std::vector<LpState*> st;
   hipGraph_t graph;
@@ -46,8 +48,8 @@ struct sovlp_batch {
   long long ticks_set;
   unsigned* hdone;           // [2] pinned host copies of the done counter
   hipEvent_t ev[2];
-  hipStream_t side[2];       // capture branches of the second and third groups
-  hipEvent_t fork, join[2];
+  hipStream_t side[3];       // capture branches of the groups after the first
+  hipEvent_t fork, join[3];
 };
 
 
@@ -68,6 +70,7 @@ struct sovlp_server {
   void* lps;                 // LpDev<T>[slots] (device)
   LpState* states;           // [slots] (device)
   long long* pcount;         // [slots][kPhaseSlots] (device)
+  long long* pticks;         // [slots][kPhaseSlots] (device)
   StepCtl* ctl;
   unsigned* done;
   char* hstage;              // pinned LpDev<T>[slots]: descriptors being copied in
@@ -89,11 +92,17 @@ constexpr int kTnnzSmall = 512;   // 64-thread workgroups: 2 x 512 x 8 B = 8 KiB
 constexpr int kPairsPerGraph = 64;     // W > 1: phase + decide launches per graph
 
 // Workgroup sizes of the batched engine, and their tile nonzero caps (16 KiB
-// of LDS for 128 threads, 32 KiB for 256)
-inline bool batch_block_ok(int block) { return block == 64 || block == 128 || block == 256; }
+// of LDS for 128 threads, 32 KiB for 256, 64 KiB for 512: the largest LPs
+// of a batch, whose phases one 256-thread workgroup runs tile after tile)
+constexpr int kBatchBlocks = 4;
+constexpr int kBatchBlock[kBatchBlocks] = {64, 128, 256, 512};
+inline bool batch_block_ok(int block) {
+  return block == 64 || block == 128 || block == 256 || block == 512;
+}
 inline int batch_tnnz(int precision, int block) {
   if (block == 64) return kTnnzSmall;
   if (block == 128) return precision == 64 ? kTnnz64 / 2 : kTnnz32 / 2;
+  if (block == 512) return precision == 64 ? kTnnz64 * 2 : kTnnz32 * 2;
   return precision == 64 ? kTnnz64 : kTnnz32;
 }
 
@@ -101,8 +110,9 @@ inline int batch_tnnz(int precision, int block) {
 template <class T>
 constexpr int run_tnnz(int block) {
   return block == 64 ? kTnnzSmall
-                     : (block == 128 ? (sizeof(T) == 8 ? kTnnz64 : kTnnz32) / 2
-                                     : (sizeof(T) == 8 ? kTnnz64 : kTnnz32));
+       : block == 128 ? (sizeof(T) == 8 ? kTnnz64 : kTnnz32) / 2
+       : block == 512 ? (sizeof(T) == 8 ? kTnnz64 : kTnnz32) * 2
+                      : (sizeof(T) == 8 ? kTnnz64 : kTnnz32);
 }
 
 // One launch of the run kernel over `count` LP descriptors with BLOCK threads
@@ -117,6 +127,7 @@ void launch_run(int block, const LpDev<T>* lps, int count, const StepCtl* ctl, u
                 int phases, hipStream_t st) {
   if (block == 64) launch_run_block<T, 64>(lps, count, ctl, done, phases, st);
   else if (block == 128) launch_run_block<T, 128>(lps, count, ctl, done, phases, st);
+  else if (block == 512) launch_run_block<T, 512>(lps, count, ctl, done, phases, st);
   else launch_run_block<T, 256>(lps, count, ctl, done, phases, st);
 }
 
@@ -170,6 +181,7 @@ size_t single_ws_T(const sovlp_lp* lp, long long hist_cap, int W) {
   cv.take<int32_t>(W + 1);
   cv.take<int32_t>(W + 1);
   cv.take<long long>(kPhaseSlots);
+  cv.take<long long>(kPhaseSlots);
   LpDev<T> L{};
   carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
   return cv.off + 256;
@@ -190,11 +202,13 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   int32_t* partA = cv.take<int32_t>(W + 1);
   int32_t* partT = cv.take<int32_t>(W + 1);
   long long* pcount = cv.take<long long>(kPhaseSlots);
+  long long* pticks = cv.take<long long>(kPhaseSlots);
   LpDev<T> L{};
   carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
   fill_lp<T>(L, *lp);
   L.partials = partials;
   L.pcount = pcount;
+  L.pticks = pticks;
   if (W > 1) {
     // equal-cost split of both tile lists (needs the pointer values at tile starts)
     std::vector<int32_t> ta(lp->ntiles_a + 1), tt(lp->ntiles_t + 1), rp(lp->m + 1), cp(lp->n + 1);
@@ -226,6 +240,7 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(done, 0, sizeof(unsigned), st));
   HIPCHK(hipMemsetAsync(pcount, 0, sizeof(long long) * kPhaseSlots, st));
+  HIPCHK(hipMemsetAsync(pticks, 0, sizeof(long long) * kPhaseSlots, st));
   HIPCHK(hipMemcpyAsync(d_lp, &L, sizeof L, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
 
@@ -241,6 +256,7 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   h->rowf = L.rowf;
   h->colf = L.colf;
   h->pcount = pcount;
+  h->pticks = pticks;
   h->hist_cap = hist_cap;
   h->ticks_set = c0.ticks;
   // record the launches of one replay into a graph on a private capture
@@ -274,6 +290,7 @@ size_t batch_ws_T(int count, const sovlp_lp* lps, long long hist_cap) {
   cv.take<StepCtl>(1);
   cv.take<unsigned>(64);
   cv.take<long long>((size_t)count * kPhaseSlots);
+  cv.take<long long>((size_t)count * kPhaseSlots);
   for (int i = 0; i < count; ++i) {
     LpDev<T> L{};
     carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
@@ -289,9 +306,9 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, cons
   // first within a group, so that the longest workgroups of a launch start
   // first and do not extend its tail; pos[i] is LP i's place in the device
   // array (every output stays indexed by i)
-  const int kBlocks[3] = {64, 128, 256};
-  std::vector<int> order, pos(count), gcount(3, 0);
-  for (int g = 0; g < 3; ++g) {
+  const int* kBlocks = kBatchBlock;
+  std::vector<int> order, pos(count), gcount(kBatchBlocks, 0);
+  for (int g = 0; g < kBatchBlocks; ++g) {
     std::vector<int> idx;
     for (int i = 0; i < count; ++i)
       if (blocks[i] == kBlocks[g]) idx.push_back(i);
@@ -303,12 +320,13 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, cons
       ++gcount[g];
     }
   }
-  if ((int)order.size() != count) return fail(1, "every LP's block must be 64, 128 or 256");
+  if ((int)order.size() != count) return fail(1, "every LP's block must be 64, 128, 256 or 512");
   Carver cv{reinterpret_cast<char*>(ws), 0, (size_t)ws_bytes};
   LpDev<T>* d_lps = cv.take<LpDev<T>>(count);
   StepCtl* ctl = cv.take<StepCtl>(1);
   unsigned* done = cv.take<unsigned>(64);
   long long* pcount = cv.take<long long>((size_t)count * kPhaseSlots);
+  long long* pticks = cv.take<long long>((size_t)count * kPhaseSlots);
   std::vector<LpDev<T>> host(count);
   h->xo.resize(count);
   h->yo.resize(count);
@@ -322,6 +340,7 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, cons
     carve_lp<T>(cv, L, lps[i].m, lps[i].n, lps[i].nnz, hist_cap);
     fill_lp<T>(L, lps[i]);
     L.pcount = pcount + (size_t)i * kPhaseSlots;
+    L.pticks = pticks + (size_t)i * kPhaseSlots;
     h->xo[i] = L.xo; h->yo[i] = L.yo; h->dout[i] = L.dout; h->st[i] = L.st;
     h->rhist[i] = L.rhist;
   }
@@ -339,21 +358,22 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, cons
   h->ctl = ctl;
   h->done = done;
   h->pcount = pcount;
+  h->pticks = pticks;
   h->ticks_set = c0.ticks;
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   // one launch per group; the groups are independent branches of the graph
   // (forked from and joined to the capture stream), so they can run side by side
   int first = -1, nside = 0;
-  for (int g = 0; g < 3; ++g)
+  for (int g = 0; g < kBatchBlocks; ++g)
     if (gcount[g] > 0 && first < 0) first = g;
   HIPCHK(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < kBatchBlocks - 1; ++k) {
     HIPCHK(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&h->join[k], hipEventDisableTiming));
   }
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
   HIPCHK(hipEventRecord(h->fork, h->cap));
-  for (int g = 0, off = 0; g < 3; off += gcount[g], ++g) {
+  for (int g = 0, off = 0; g < kBatchBlocks; off += gcount[g], ++g) {
     if (gcount[g] == 0) continue;
     if (g == first) {
       launch_run<T>(kBlocks[g], d_lps + off, gcount[g], ctl, done, kPhasesPerLaunch, h->cap);
@@ -443,6 +463,7 @@ size_t server_ws_T(int slots, int team) {
   cv.take<LpDev<T>>(slots);
   cv.take<LpState>(slots);
   cv.take<long long>((size_t)slots * kPhaseSlots);
+  cv.take<long long>((size_t)slots * kPhaseSlots);
   cv.take<StepCtl>(1);
   cv.take<unsigned>(64);
   if (team > 1) {
@@ -468,6 +489,7 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
   LpDev<T>* d_lps = cv.take<LpDev<T>>(slots);
   LpState* states = cv.take<LpState>(slots);
   long long* pcount = cv.take<long long>((size_t)slots * kPhaseSlots);
+  long long* pticks = cv.take<long long>((size_t)slots * kPhaseSlots);
   StepCtl* ctl = cv.take<StepCtl>(1);
   unsigned* done = cv.take<unsigned>(64);
   h->team = team;
@@ -484,6 +506,7 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
   h->lps = d_lps;
   h->states = states;
   h->pcount = pcount;
+  h->pticks = pticks;
   h->ctl = ctl;
   h->done = done;
   h->busy.assign(slots, 0);
@@ -500,6 +523,7 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
     stage[i] = LpDev<T>{};
     stage[i].st = states + i;
     stage[i].pcount = pcount + (size_t)i * kPhaseSlots;
+    stage[i].pticks = pticks + (size_t)i * kPhaseSlots;
     h->hinit[i] = LpState{};
     h->hinit[i].phase = PH_DONE;
     h->hinit[i].status = ST_RUNNING;
@@ -515,6 +539,7 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
   HIPCHK(hipMemcpyAsync(d_lps, stage, sizeof(LpDev<T>) * (size_t)slots, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(states, h->hinit, sizeof(LpState) * (size_t)slots, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(pcount, 0, sizeof(long long) * kPhaseSlots * (size_t)slots, st));
+  HIPCHK(hipMemsetAsync(pticks, 0, sizeof(long long) * kPhaseSlots * (size_t)slots, st));
   HIPCHK(hipMemcpyAsync(ctl, &c0, sizeof c0, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(done, 0, sizeof(unsigned), st));
   HIPCHK(hipStreamSynchronize(st));
@@ -547,6 +572,7 @@ int server_admit_T(sovlp_server* h, int slot, const sovlp_lp* lp, const double* 
   fill_lp<T>(L, *lp);
   L.st = h->states + slot;   // the state lives in the server's table, not the LP's workspace
   L.pcount = h->pcount + (size_t)slot * kPhaseSlots;
+  L.pticks = h->pticks + (size_t)slot * kPhaseSlots;
   if (h->team > 1) {
     // equal-cost split of both tile lists over the team (host plan from the
     // device copies of the tile starts and row pointers)
@@ -581,6 +607,7 @@ int server_admit_T(sovlp_server* h, int slot, const sovlp_lp* lp, const double* 
   HIPCHK(hipMemcpyAsync(d_lps + slot, &stage[slot], sizeof(LpDev<T>), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(h->states + slot, &h->hinit[slot], sizeof(LpState), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(h->pcount + (size_t)slot * kPhaseSlots, 0, sizeof(long long) * kPhaseSlots, st));
+  HIPCHK(hipMemsetAsync(h->pticks + (size_t)slot * kPhaseSlots, 0, sizeof(long long) * kPhaseSlots, st));
   h->xo[slot] = L.xo;
   h->yo[slot] = L.yo;
   h->dout[slot] = L.dout;

@@ -186,6 +186,14 @@ int sovlp_pdlp_phase_counts(sovlp_pdlp* h, int64_t* counts, sovlp_stream_t strea
   return 0;
 }
 
+int sovlp_pdlp_phase_ticks(sovlp_pdlp* h, int64_t* ticks, sovlp_stream_t stream) {
+  if (!h || !ticks) return fail(1, "null argument");
+  HIPCHK(hipMemcpyAsync(ticks, h->pticks, sizeof(long long) * kPhaseSlots,
+                        hipMemcpyDeviceToHost, S(stream)));
+  HIPCHK(hipStreamSynchronize(S(stream)));
+  return 0;
+}
+
 int sovlp_pdlp_destroy(sovlp_pdlp* h) {
   if (!h) return 0;
   for (hipEvent_t e : h->ev)
@@ -201,7 +209,7 @@ int sovlp_pdlp_destroy(sovlp_pdlp* h) {
 
 int sovlp_batch_geometry(int precision, int block, int* tile_rows, int* tile_nnz) {
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
+  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128, 256 or 512");
   *tile_rows = kRowsPerLane * block;
   *tile_nnz = batch_tnnz(precision, block);
   return 0;
@@ -224,7 +232,7 @@ int sovlp_batch_create_mixed(int count, const sovlp_lp* lps, const sovlp_params*
     return fail(1, "bad batch arguments");
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
   for (int i = 0; i < count; ++i)
-    if (!batch_block_ok(blocks[i])) return fail(1, "block must be 64, 128 or 256");
+    if (!batch_block_ok(blocks[i])) return fail(1, "block must be 64, 128, 256 or 512");
   if (int rc = check_params(params, precision)) return rc;
   sovlp_batch* h = new sovlp_batch{};
   int rc = precision == 64
@@ -244,7 +252,7 @@ int sovlp_batch_create(int count, const sovlp_lp* lps, const sovlp_params* param
                        int precision, int block, void* workspace, int64_t workspace_bytes,
                        int64_t history_cap, sovlp_stream_t stream, sovlp_batch** out) {
   if (count < 1) return fail(1, "bad batch arguments");
-  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
+  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128, 256 or 512");
   std::vector<int32_t> blocks(count, block);
   return sovlp_batch_create_mixed(count, lps, params, precision, blocks.data(), workspace,
                                   workspace_bytes, history_cap, stream, out);
@@ -316,9 +324,17 @@ int sovlp_batch_phase_counts(sovlp_batch* h, int64_t* counts, sovlp_stream_t str
   return 0;
 }
 
+int sovlp_batch_phase_ticks(sovlp_batch* h, int64_t* ticks, sovlp_stream_t stream) {
+  if (!h || !ticks) return fail(1, "null argument");
+  HIPCHK(hipMemcpyAsync(ticks, h->pticks, sizeof(long long) * kPhaseSlots * (size_t)h->count,
+                        hipMemcpyDeviceToHost, S(stream)));
+  HIPCHK(hipStreamSynchronize(S(stream)));
+  return 0;
+}
+
 int sovlp_batch_destroy(sovlp_batch* h) {
   if (!h) return 0;
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < kBatchBlocks - 1; ++k) {
     if (h->side[k]) (void)hipStreamDestroy(h->side[k]);
     if (h->join[k]) (void)hipEventDestroy(h->join[k]);
   }
@@ -365,7 +381,7 @@ static int server_create(int slots, int team, const sovlp_params* params, int pr
   if (slots < 1 || team < 1 || !params || !workspace || !out)
     return fail(1, "bad server arguments");
   if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
-  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128 or 256");
+  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128, 256 or 512");
   if (phases_per_launch < 1 || phases_per_launch > 4096)
     return fail(1, "phases_per_launch must be in [1, 4096]");
   if ((long long)slots * team > (1LL << 24)) return fail(1, "slots x team too large");

@@ -110,15 +110,21 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
   __shared__ Params sp;                  // the parameters, read once per launch
   __shared__ long long kstop;
   __shared__ int pc[kPhaseSlots];        // phase counts of this launch
+  __shared__ long long pt[kPhaseSlots];  // device clock ticks of each phase in this launch
   const LpDev<T>& L = lps[blockIdx.x];
   if (threadIdx.x == 0) {
     ss = *L.st;
     sp = ctl->P;
     kstop = ctl->kstop;
   }
-  if (threadIdx.x < kPhaseSlots) pc[threadIdx.x] = 0;
+  if (threadIdx.x < kPhaseSlots) {
+    pc[threadIdx.x] = 0;
+    pt[threadIdx.x] = 0;
+  }
   __syncthreads();
   if (uniform(ss.phase) == PH_DONE) return;
+  // thread 0: when the current phase started (the end of the previous one)
+  long long tp = threadIdx.x == 0 ? (long long)wall_clock64() : 0;
   for (int it = 0; it < max_phases; ++it) {
     const int ph = uniform(ss.phase);
     if (ph == PH_DONE) break;
@@ -140,11 +146,17 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
       } else {
         transition_in_lds(L, ph, red + (BLOCK / kWave) * kMaxRed, ctl, sp, done, pc, ss);
       }
+      const long long now = (long long)wall_clock64();
+      pt[ph] += now - tp;
+      tp = now;
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) *L.st = ss;
-  if (threadIdx.x < kPhaseSlots && pc[threadIdx.x] != 0) L.pcount[threadIdx.x] += pc[threadIdx.x];
+  if (threadIdx.x < kPhaseSlots && pc[threadIdx.x] != 0) {
+    L.pcount[threadIdx.x] += pc[threadIdx.x];
+    L.pticks[threadIdx.x] += pt[threadIdx.x];
+  }
 }
 
 // W workgroups share one LP's phase; each writes its partial sums.
@@ -204,7 +216,10 @@ __global__ void state_reset_kernel(const LpDev<T>* __restrict__ lps, int count, 
   s.phase = PH_INIT;
   s.status = ST_RUNNING;
   *lps[i].st = s;
-  for (int k = 0; k < kPhaseSlots; ++k) lps[i].pcount[k] = 0;
+  for (int k = 0; k < kPhaseSlots; ++k) {
+    lps[i].pcount[k] = 0;
+    lps[i].pticks[k] = 0;
+  }
 }
 
 

@@ -159,6 +159,7 @@ struct LpDev {
   double* rhist;             // [kMaxRounds][5] per round: k, pres, dres, gap, pobj
   LpState* st;
   long long* pcount;         // [kPhaseSlots] phase counters
+  long long* pticks;         // [kPhaseSlots] device clock ticks spent in each phase (run kernel)
   double* partials;          // [W][kMaxRed] per-workgroup partial sums (W-way phases)
   // warm start (served resolves; null / 0 = the reference's cold start
   // x0 = clip(0), y0 = 0, omega0 from the norms): x and y in the units and

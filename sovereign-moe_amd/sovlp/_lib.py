@@ -102,6 +102,8 @@ _SIGS = {
     "sovlp_pdlp_info": (C.c_int, [P, C.POINTER(sovlp_info), P]),
     "sovlp_pdlp_outputs": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
     "sovlp_pdlp_width": (C.c_int, [P, C.POINTER(C.c_int)]),
+    "sovlp_pdlp_phase_ticks": (C.c_int, [P, P, P]),
+    "sovlp_batch_phase_ticks": (C.c_int, [P, P, P]),
     "sovlp_pdlp_scaling": (C.c_int, [P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "sovlp_phases_per_launch": (C.c_int, [C.POINTER(C.c_int)]),
     "sovlp_pdlp_refinement": (C.c_int, [P, C.POINTER(P)]),

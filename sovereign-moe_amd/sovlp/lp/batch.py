@@ -191,6 +191,14 @@ class BatchEngine:
         _lib.call("sovlp_batch_phase_counts", self.h, out.ctypes.data, self.stream)
         return out
 
+    def phase_ticks(self) -> np.ndarray:
+        """(count, slots) device clock ticks each LP spent in each phase."""
+        slots = C.c_int()
+        _lib.call("sovlp_phase_slots", C.byref(slots))
+        out = np.zeros((self.count, slots.value), dtype=np.int64)
+        _lib.call("sovlp_batch_phase_ticks", self.h, out.ctypes.data, self.stream)
+        return out
+
     def sizes(self) -> list[tuple[int, int, int, int, int]]:
         """(m, n, nnz, tiles of A, tiles of A^T) of every LP."""
         return [(m, n, nnz, ta, tt) for (m, n, nnz, ta, tt, _o, _f) in self.db.meta]

@@ -100,14 +100,14 @@ def test_small_workgroups_agree_with_large():
 
 
 def test_autotuned_mixed_configurations_in_one_batch():
-    """Per-LP kernel configurations from an autotuner table (64, 128 and 256
+    """Per-LP kernel configurations from an autotuner table (64, 128 and 512
     threads, different tile caps) in one batch: every LP reaches its closed
     form, and an LP's result is identical alone and in the mix."""
     from sovlp.lp.autotune import Autotuner, bucket_of, config_of
     print("[gpu] mixed configurations", flush=True)
     t = Autotuner(32)
     t.table = {0: config_of(32, 64, 4, 1.0), 1: config_of(32, 128, 1, 1.0),
-               2: config_of(32, 256, 4, 0.5), 3: config_of(32, 128, 4, 1.0)}
+               2: config_of(32, 512, 4, 0.5), 3: config_of(32, 128, 4, 1.0)}
     kos = _batch(12, base=9000)
     assert len({bucket_of(k.problem) for k in kos}) >= 2
     par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6)
