@@ -32,9 +32,10 @@ Beside the headline number the JSON line carries:
                 FP64 and FP32, in the given and the locality column order
                 (HIP events, algorithmic GB/s against 8 TB/s)
   large_lp      the 200k x 1M skewed LP solved once in FP64 (rel KKT 1e-6)
-                and once in FP32 + iterative refinement: status, iterations,
-                device time, objective error against the closed form, and the
-                refinement record
+                and once in FP32 + iterative refinement, with the engine's
+                defaults for an LP of this size (locality column order, rounds
+                at 1e-3): status, iterations, device time, objective error
+                against the closed form, and the refinement record
   bnr           batched node relaxation, 128 nodes of a 500 x 2000 LP
                 (optional, --bnr)
   certify       the certificate of a point of the 200k x 1M LP (optional,
@@ -101,8 +102,9 @@ def parse():
                     help="skip the 200k x 1M LP solves (FP64, FP32 + refinement)")
     ap.add_argument("--panels", action="store_true",
                     help="SpMV section: also A x as eight XCD column panels")
-    ap.add_argument("--large-lp-locality", action="store_true",
-                    help="large LP: also solve with the columns in locality order")
+    ap.add_argument("--large-lp-given", action="store_true",
+                    help="large LP: also solve in the given column order at the "
+                         "small-LP refinement tolerance")
     ap.add_argument("--serve", action="store_true",
                     help="also run a Poisson stream of blending LPs through the serving loop")
     ap.add_argument("--serve-rate", type=float, default=10.0)
@@ -285,9 +287,9 @@ def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: in
 
 
 def slowest_lps(eng, sols, lat, problems, k: int = 8) -> list:
-    """The LPs that set the step: index, size, kernel block, iterations,
-    refinement rounds, status, start and latency on the device clock (ms from
-    the step's first LP start)."""
+    """The LPs that set the step, latest finish first: index, size, kernel
+    block, iterations, refinement rounds, status, start, latency and finish on
+    the device clock (ms from the step's first LP start)."""
     import ctypes
     import numpy as np
     from sovlp import _lib
@@ -295,15 +297,17 @@ def slowest_lps(eng, sols, lat, problems, k: int = 8) -> list:
     _lib.call("sovlp_clock_rate", 0, ctypes.byref(hz))
     infos = eng.infos()
     t0 = min(int(infos[i].start_tick) for i in range(eng.count))
+    start = np.array([(int(infos[i].start_tick) - t0) / hz.value for i in range(eng.count)])
+    end = start + np.asarray(lat, dtype=np.float64)
     out = []
-    for i in np.argsort(lat)[::-1][:k]:
+    for i in np.argsort(end)[::-1][:k]:
         i = int(i)
         p = problems[i]
         out.append({"lp": i, "m": p.m, "n": p.n, "nnz": int(p.A.nnz),
                     "block": int(eng.db.blocks[i]), "iterations": int(sols[i].iterations),
                     "rounds": len(sols[i].info.refinement), "status": sols[i].status.name,
-                    "start_ms": (int(infos[i].start_tick) - t0) / hz.value * 1e3,
-                    "latency_ms": float(lat[i]) * 1e3})
+                    "start_ms": start[i] * 1e3, "latency_ms": float(lat[i]) * 1e3,
+                    "end_ms": end[i] * 1e3})
     return out
 
 
@@ -452,35 +456,43 @@ def gpu_configs0(reps: int = 3) -> dict:
             "rel_obj_err": abs(inf.objective - ko.f_star) / max(1.0, abs(ko.f_star))}
 
 
-def large_lp_section(ko, locality: bool = False) -> dict:
+def large_lp_section(ko, given: bool = False) -> dict:
     """The 200k x 1M skewed LP solved from scratch: FP64 (the reference's
-    arithmetic) and FP32 + iterative refinement, rel KKT 1e-6 (feas_cap 1e-6);
-    time on the device clock and the wall, iterations, objective error against
-    the closed form, and for FP32 the refinement record."""
+    arithmetic) and FP32 + iterative refinement, rel KKT 1e-6 (feas_cap 1e-6),
+    with the engine's defaults for an LP of this size (columns in locality
+    order, refinement rounds at 1e-3), and with ``given`` also in the caller's
+    column order at the small-LP round tolerance; time on the device clock and
+    the wall, iterations, objective error against the closed form, and for
+    FP32 the refinement record."""
     from sovlp.lp.pdlp import PDLPParams, solve_pdlp
     f = ko.f_star
     rel = lambda v: abs(v - f) / max(1.0, abs(f))  # noqa: E731
     out = {"lp": f"{ko.problem.m} x {ko.problem.n}, {ko.problem.A.nnz} nonzeros, Zipf column "
                  f"degrees (s=1.3, 2..5000)", "tolerance": 1e-6}
-    orders = ["given"] + (["locality"] if locality else [])
+    orders = ["auto"] + (["given"] if given else [])
     for order in orders:
         for prec in (64, 32):
+            extra = {} if order == "auto" else {"refine_eps": 3e-4}
             par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6, device="gpu",
-                             precision=prec, time_limit=240.0, column_order=order)
+                             precision=prec, time_limit=240.0, column_order=order, **extra)
             t = time.perf_counter()
             sol = solve_pdlp(ko.problem, par)
             wall = time.perf_counter() - t
+            nnz = int(ko.problem.A.nnz)
             r = {"status": sol.status.name, "iterations": int(sol.iterations),
                  "restarts": int(sol.info.restarts), "device_s": sol.info.device_seconds,
                  "wall_s": wall, "us_per_iteration": sol.info.device_seconds /
                  max(1, sol.iterations) * 1e6, "workgroups": sol.info.workgroups,
+                 "column_order": par.resolved_column_order(nnz),
                  "rel_obj_err": rel(sol.objective)}
+            if prec == 32:
+                r["round_eps"] = par.resolved_refine(32, nnz)[1]
             if sol.info.refinement:
                 r["refinement"] = [{"iterations": int(q[0]), "primal_res": q[1], "dual_res": q[2],
                                     "gap": q[3], "rel_obj_err": rel(q[4])}
                                    for q in sol.info.refinement]
             key = ("fp64" if prec == 64 else "fp32_refined") + \
-                ("" if order == "given" else "_locality")
+                ("" if order == "auto" else "_given")
             out[key] = r
             log(f"large LP {key}: {r}")
     return out
@@ -582,7 +594,7 @@ def main():
 
     if a.only:
         sec = {"spmv": spmv_section, "large_lp": lambda: large_lp_section(
-            large(), a.large_lp_locality), "bnr": bnr_rate, "certify": lambda: certify_rate(k=large()),
+            large(), a.large_lp_given), "bnr": bnr_rate, "certify": lambda: certify_rate(k=large()),
                "serve": lambda: serve_section(a), "copy": copy_calibration}[a.only]()
         print(json.dumps({a.only: sec}), flush=True)
         return
@@ -734,7 +746,7 @@ def main():
     if not a.no_spmv:
         out["spmv"] = spmv_section()
     if not a.no_large_lp:
-        out["large_lp"] = large_lp_section(large(), a.large_lp_locality)
+        out["large_lp"] = large_lp_section(large(), a.large_lp_given)
     if a.serve:
         out["serve"] = serve_section(a)
         log(f"serve: {out['serve']}")

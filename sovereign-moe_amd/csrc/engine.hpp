@@ -362,9 +362,14 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, cons
   h->ticks_set = c0.ticks;
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   // one launch per group; the groups are independent branches of the graph
-  // (forked from and joined to the capture stream), so they can run side by side
+  // (forked from and joined to the capture stream), so they can run side by
+  // side.  The largest workgroups are launched first: groups of smaller ones
+  // dispatched ahead of them would take the free slots and hold back the
+  // batch's largest LPs, which set the step.
+  int goff[kBatchBlocks];
+  for (int g = 0, off = 0; g < kBatchBlocks; off += gcount[g], ++g) goff[g] = off;
   int first = -1, nside = 0;
-  for (int g = 0; g < kBatchBlocks; ++g)
+  for (int g = kBatchBlocks - 1; g >= 0; --g)
     if (gcount[g] > 0 && first < 0) first = g;
   HIPCHK(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
   for (int k = 0; k < kBatchBlocks - 1; ++k) {
@@ -373,7 +378,8 @@ int batch_create_T(int count, const sovlp_lp* lps, const sovlp_params* prm, cons
   }
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
   HIPCHK(hipEventRecord(h->fork, h->cap));
-  for (int g = 0, off = 0; g < kBatchBlocks; off += gcount[g], ++g) {
+  for (int g = kBatchBlocks - 1; g >= 0; --g) {
+    const int off = goff[g];
     if (gcount[g] == 0) continue;
     if (g == first) {
       launch_run<T>(kBlocks[g], d_lps + off, gcount[g], ctl, done, kPhasesPerLaunch, h->cap);

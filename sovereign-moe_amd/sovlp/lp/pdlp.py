@@ -29,7 +29,16 @@ from ..core.backend import get_backend
 from ..core.problem import ObjSense, Problem, Solution, Status
 from ..core.sparse import locality_order, relabel_columns
 
-__all__ = ["PDLPParams", "PDLPInfo", "solve_pdlp", "DeviceLP", "upload_problem"]
+__all__ = ["LARGE_NNZ", "PDLPParams", "PDLPInfo", "solve_pdlp", "DeviceLP", "upload_problem"]
+
+
+# An LP of this many nonzeros or more is "large": its columns default to the
+# locality order and its refinement rounds to a looser tolerance.  On the
+# 200k x 1M skewed LP (8.3M nonzeros, seeds 1-3, device seconds) FP32 +
+# refinement took 1.08 / 1.62 / 1.36 s with both, against 3.58 / 2.26 / 3.64 s
+# in the given order at 3e-4; FP64 took 1.94 / 1.58 / 2.02 s (locality) and
+# 1.87 / 2.29 / 2.35 s (given).
+LARGE_NNZ = 1 << 20
 
 
 @dataclass
@@ -55,9 +64,11 @@ class PDLPParams:
     chunk_iters: int = 8192
     # iterative refinement of the FP32 engine (None: 6 rounds at precision
     # 32, none at 64; each round to relative KKT max(eps_rel, 3e-4), which
-    # FP32 iterates reach; the rounds together reach eps and feas_cap).  The
-    # stall lengths trade LPs finished for time: on 4096 uneven LPs the
-    # defaults finish 99.5 % in 0.82 s, refine_stall_open=1024 98.2 % in 0.33 s.
+    # FP32 iterates reach, or max(eps_rel, 1e-3) for an LP of LARGE_NNZ or
+    # more nonzeros, whose rounds then stop sooner and zoom further; the
+    # rounds together reach eps and feas_cap).  The stall lengths trade LPs
+    # finished for time: on 4096 uneven LPs the defaults finish 99.5 % in
+    # 0.82 s, refine_stall_open=1024 98.2 % in 0.33 s.
     refine_rounds: int | None = None
     refine_eps: float | None = None
     refine_tol: float = 1e-9
@@ -71,19 +82,30 @@ class PDLPParams:
     # gathers of A x on the long columns into a short prefix of x; x and the
     # reduced costs come back in the caller's order. A x is bit for bit that of
     # the given order; A^T y (whose tiles move with the columns) and the sums
-    # over columns agree to rounding.
-    column_order: str = "given"
+    # over columns agree to rounding.  "auto": "locality" for an LP of
+    # LARGE_NNZ or more nonzeros (x no longer fits one XCD's L2 and the
+    # gathers dominate A x), "given" below.
+    column_order: str = "auto"
 
-    def resolved_refine(self, precision: int | None = None) -> tuple[int, float]:
+    def resolved_refine(self, precision: int | None = None,
+                        nnz: int | None = None) -> tuple[int, float]:
+        """(rounds, per-round tolerance) of the refinement; ``nnz``: the LP's
+        nonzeros (None: a small LP)."""
         prec = int(self.precision if precision is None else precision)
         rounds = self.refine_rounds
         if rounds is None:
             rounds = 6 if prec == 32 else 0
-        eps = self.refine_eps if self.refine_eps is not None else max(self.eps_rel, 3e-4)
+        floor = 1e-3 if nnz is not None and nnz >= LARGE_NNZ else 3e-4
+        eps = self.refine_eps if self.refine_eps is not None else max(self.eps_rel, floor)
         return int(rounds), float(eps)
 
-    def to_c(self, precision: int | None = None) -> _lib.sovlp_params:
-        rounds, eps = self.resolved_refine(precision)
+    def resolved_column_order(self, nnz: int) -> str:
+        if self.column_order == "auto":
+            return "locality" if nnz >= LARGE_NNZ else "given"
+        return self.column_order
+
+    def to_c(self, precision: int | None = None, nnz: int | None = None) -> _lib.sovlp_params:
+        rounds, eps = self.resolved_refine(precision, nnz)
         return _lib.sovlp_params(self.eps_abs, self.eps_rel, self.feas_cap, int(self.max_iter),
                                  float(self.time_limit), int(self.check_every),
                                  int(self.stepsize_every), self.beta_sufficient,
@@ -212,7 +234,7 @@ class Engine:
         _lib.call("sovlp_pdlp_workspace_bytes", C.byref(dlp.desc), prec, self.hist_cap,
                   C.byref(nbytes))
         self.ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device="cuda")
-        self.cparams = params.to_c(prec)
+        self.cparams = params.to_c(prec, int(dlp.desc.nnz))
         h = C.c_void_p()
         if params.team not in _TEAMS:
             raise ValueError(f"team must be one of {sorted(_TEAMS)}")
@@ -287,12 +309,12 @@ def solve_pdlp(prob: Problem, params: PDLPParams | None = None) -> Solution:
     t0 = time.perf_counter()
     if params.scaling != "pdlp":
         raise ValueError("only scaling='pdlp' (Ruiz + Pock-Chambolle) is provided")
-    if params.column_order not in ("given", "locality"):
-        raise ValueError("column_order is 'given' or 'locality'")
+    if params.column_order not in ("auto", "given", "locality"):
+        raise ValueError("column_order is 'auto', 'given' or 'locality'")
     bk = get_backend(params.device)          # raises without a usable GPU
     new = None
     v0 = None
-    if params.column_order == "locality" and prob.n > 0:
+    if params.resolved_column_order(int(prob.A.nnz)) == "locality" and prob.n > 0:
         import torch
         new = locality_order(prob.A)
         order = np.argsort(new)

@@ -186,6 +186,26 @@ def test_locality_order_relabels_without_changing_any_sum():
         relabel_columns(A, np.zeros(n, dtype=np.int64))
 
 
+def test_large_lp_defaults():
+    """LPs of LARGE_NNZ nonzeros or more default to the locality column order
+    and refinement rounds at 1e-3; smaller ones to the given order and 3e-4;
+    explicit settings win."""
+    from sovlp.lp.pdlp import LARGE_NNZ, PDLPParams
+    p = PDLPParams(eps_abs=1e-6, eps_rel=1e-6)
+    assert p.resolved_column_order(LARGE_NNZ - 1) == "given"
+    assert p.resolved_column_order(LARGE_NNZ) == "locality"
+    assert p.resolved_refine(32) == (6, 3e-4)
+    assert p.resolved_refine(32, LARGE_NNZ - 1) == (6, 3e-4)
+    assert p.resolved_refine(32, LARGE_NNZ) == (6, 1e-3)
+    assert p.resolved_refine(64, LARGE_NNZ) == (0, 1e-3)
+    q = PDLPParams(eps_abs=1e-6, eps_rel=1e-6, column_order="given", refine_eps=2e-4)
+    assert q.resolved_column_order(LARGE_NNZ) == "given"
+    assert q.resolved_refine(32, LARGE_NNZ) == (6, 2e-4)
+    assert PDLPParams(eps_rel=5e-3).resolved_refine(32, LARGE_NNZ)[1] == 5e-3
+    assert q.to_c(32, LARGE_NNZ).refine_eps == 2e-4
+    assert p.to_c(32, LARGE_NNZ).refine_eps == 1e-3
+
+
 def test_column_order_is_checked():
     from sovlp.lp.pdlp import PDLPParams, solve_pdlp
     from sovlp.workloads import small_lp
