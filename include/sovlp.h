@@ -524,6 +524,97 @@ int sovlp_server_clock_hz(sovlp_server *h, double *hz);
 
 int sovlp_server_destroy(sovlp_server *h);
 
+/* ---------------------------------------------------------------------------
+ * Crossover (host code): an optimal basis from a PDLP point.
+ *
+ * The variables are z = [x ; s] with A x - s = 0 (one logical per row, bounded
+ * by the row bounds), the basis convention of the reference
+ * (src/sovopt/lp/basis.py:75-79, 203-215): status codes BASIC 0, AT_LOWER 1,
+ * AT_UPPER 2, FREE 3, FIXED 4, one per variable (n structurals, then m
+ * logicals).  The basis is factorised densely: at most 8192 rows (error 4).
+ * ------------------------------------------------------------------------- */
+
+/* An LP on the host in CSC form, objective minimised; +-1e30 = no bound. */
+typedef struct {
+  int32_t m, n;
+  int64_t nnz;
+  const int64_t *cp;    /* n + 1 column pointers */
+  const int32_t *ci;    /* row of each nonzero */
+  const double *cx;     /* value of each nonzero */
+  const double *c;      /* n costs */
+  const double *col_lb, *col_ub;   /* n */
+  const double *row_lb, *row_ub;   /* m */
+} sovlp_host_lp;
+
+/* Fields of the reference's SimplexParams (src/sovopt/lp/simplex.py:94-187)
+ * that this simplex reads. */
+typedef struct {
+  int64_t max_iter;
+  double time_limit;        /* seconds */
+  double feas_tol;          /* primal feasibility, relative to 1 + |bound| */
+  double opt_tol;           /* reduced-cost optimality, relative to 1 + |c_j| */
+  double pivot_tol;         /* smallest |alpha| the ratio test takes */
+  int32_t refactor_freq;    /* pivots between refactorisations */
+  int32_t perturb_after;    /* degenerate pivots before the costs are perturbed (0: never) */
+  double perturb_scale;
+} sovlp_simplex_params;
+
+typedef struct {
+  int32_t status;           /* Status code: 0 OPTIMAL, 1 INFEASIBLE, 2 UNBOUNDED,
+                               4 iteration limit, 5 time limit, 8 NUMERICAL */
+  int32_t repairs;          /* columns the factorisations replaced by logicals */
+  int64_t iterations;       /* pivots + bound flips */
+  int64_t pivots;
+  int64_t phase1_iterations;
+  int64_t superbasics;      /* sovlp_crossover: nonbasics it started between their bounds */
+  double objective;         /* c x */
+  double primal_infeasibility;
+  double dual_infeasibility;
+  double seconds;
+} sovlp_simplex_result;
+
+/* Bounded primal revised simplex (phase 1 on the sum of infeasibilities, then
+ * phase 2; Dantzig pricing, Harris ratio test, bound flips, cost perturbation
+ * on degenerate stalls).  `warm_status` (n + m entries, exactly m BASIC, or
+ * NULL: the all-logical basis with statuses from the costs) is the starting
+ * basis.  Outputs: x (n), y (m; c - A^T y are the reduced costs, y the duals
+ * of the rows), d (n reduced costs), the final status vector (n + m).
+ * Replaces the simplex the reference's crossover finishes with
+ * (src/sovopt/lp/simplex.py:1181-1300, solve_simplex(prob, params, warm_basis)). */
+int sovlp_simplex_solve(const sovlp_host_lp *lp, const sovlp_simplex_params *params,
+                        const uint8_t *warm_status, double *x, double *y, double *d,
+                        uint8_t *status, sovlp_simplex_result *result);
+
+/* A basis for the point x (n) and, if not NULL, duals y (m, tie-break only):
+ * each variable of [x ; A x] scored by its distance to its nearest bound over
+ * its range, then the structurals off their bounds pushed into the
+ * all-logical basis best first, each pivoting on its largest entry in a row
+ * that still holds its logical (candidates whose entries there are all at
+ * most pivot_tol are skipped), so the basis is nonsingular by construction.
+ * Writes the n + m statuses and the number of structurals placed.
+ * Replaces basis_from_point (src/sovopt/lp/crossover.py:80-188). */
+int sovlp_basis_from_point(const sovlp_host_lp *lp, const double *x, const double *y,
+                           double pivot_tol, double tol_fixed, uint8_t *status,
+                           int32_t *placed);
+
+/* The crossover (src/sovopt/lp/crossover.py:192-222, crossover(prob, x, y)):
+ * the basis of sovlp_basis_from_point for x0 (duals y0, may be NULL), then
+ * the simplex from it with every nonbasic variable that x0 puts strictly
+ * between its bounds held at that value (a superbasic) rather than moved to
+ * a bound first; the simplex pushes those to bounds or into the basis
+ * itself.  Starting at the point keeps the basic values at the point's, so
+ * phase 1 has only the point's own infeasibility to remove.  Outputs as
+ * sovlp_simplex_solve. */
+int sovlp_crossover(const sovlp_host_lp *lp, const sovlp_simplex_params *params,
+                    const double *x0, const double *y0, double *x, double *y, double *d,
+                    uint8_t *status, sovlp_simplex_result *result);
+
+/* Factorises the basis a status vector names (exactly m BASIC) and reports
+ * how many of its columns had to be replaced by logicals (0: nonsingular).
+ * The check of the reference's Basis.factorize / n_repairs
+ * (tests/test_crossover.py:47-70). */
+int sovlp_basis_repairs(const sovlp_host_lp *lp, const uint8_t *status, int32_t *repairs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,24 @@ class sovlp_bnr_batch(C.Structure):
                 ("dual_sign", F64)]
 
 
+class sovlp_host_lp(C.Structure):
+    _fields_ = [("m", I32), ("n", I32), ("nnz", I64), ("cp", P), ("ci", P), ("cx", P),
+                ("c", P), ("col_lb", P), ("col_ub", P), ("row_lb", P), ("row_ub", P)]
+
+
+class sovlp_simplex_params(C.Structure):
+    _fields_ = [("max_iter", I64), ("time_limit", F64), ("feas_tol", F64), ("opt_tol", F64),
+                ("pivot_tol", F64), ("refactor_freq", I32), ("perturb_after", I32),
+                ("perturb_scale", F64)]
+
+
+class sovlp_simplex_result(C.Structure):
+    _fields_ = [("status", I32), ("repairs", I32), ("iterations", I64), ("pivots", I64),
+                ("phase1_iterations", I64), ("superbasics", I64), ("objective", F64),
+                ("primal_infeasibility", F64),
+                ("dual_infeasibility", F64), ("seconds", F64)]
+
+
 # name -> (restype, argtypes); every entry point of include/sovlp.h
 _SIGS = {
     "sovlp_last_error": (C.c_char_p, []),
@@ -152,6 +170,14 @@ _SIGS = {
     "sovlp_certify_f64": (C.c_int, [C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P, P, F64,
                                     C.c_int, P, P, F64, P, I64, P, P,
                                     C.POINTER(sovlp_certificate), P]),
+    "sovlp_simplex_solve": (C.c_int, [C.POINTER(sovlp_host_lp),
+                                      C.POINTER(sovlp_simplex_params), P, P, P, P, P,
+                                      C.POINTER(sovlp_simplex_result)]),
+    "sovlp_basis_from_point": (C.c_int, [C.POINTER(sovlp_host_lp), P, P, F64, F64, P,
+                                         C.POINTER(I32)]),
+    "sovlp_basis_repairs": (C.c_int, [C.POINTER(sovlp_host_lp), P, C.POINTER(I32)]),
+    "sovlp_crossover": (C.c_int, [C.POINTER(sovlp_host_lp), C.POINTER(sovlp_simplex_params),
+                                  P, P, P, P, P, P, C.POINTER(sovlp_simplex_result)]),
 }
 
 

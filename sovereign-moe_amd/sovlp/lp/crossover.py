@@ -1,0 +1,102 @@
+"""Crossover: an optimal basis from the PDLP point.
+
+Reference: ``basis_from_point`` and ``crossover``
+(``src/sovopt/lp/crossover.py:80-222``).  A first-order solve stops at a
+point that is optimal to its tolerance but not a vertex, so it carries no
+basis (no ranging, no warm start for a re-solve).  The crossover scores every
+variable of ``[x ; A x]`` by its distance to its nearest bound over its range,
+pushes the structurals that sit off their bounds into the all-logical basis
+best first -- each by an FTRAN and a pivot on its largest entry in a row that
+still holds its logical, so the basis is nonsingular by construction (ranking
+alone gives a singular set) -- and hands that basis to the revised simplex,
+which terminates only at an optimal basis.  Both steps run natively
+(``crossover.cpp`` in ``libsovlp.so``: ``sovlp_basis_from_point`` is the
+identification alone, ``sovlp_crossover`` the whole crossover).
+
+One difference from the reference: its simplex starts from the statuses
+alone, every nonbasic variable moved to its nearer bound, which throws the
+basic values far from the point when more variables sit between their bounds
+than the basis holds (a first-order point on a degenerate optimal face).  Here
+those variables start at the point's values (superbasic) and the simplex
+pushes them to bounds or into the basis itself, so its phase 1 removes only
+the point's own infeasibility.  On the hourly blending plan from a
+20k-iteration PDLP point: 6913 iterations (4175 in phase 1) from the statuses
+alone, 1118 (109 in phase 1) this way, against 4377 for a cold simplex.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from .. import _lib
+from ..core.problem import ObjSense, Problem, Solution
+from .basis import host_lp
+from .simplex import SimplexParams, result_solution
+
+__all__ = ["basis_from_point", "crossover"]
+
+
+def basis_from_point(prob: Problem, x, y=None, params=None,
+                     tol_fixed: float = 1e-12) -> np.ndarray:
+    """The ``n + m`` status vector of a basis for the point ``x`` (``y``, the
+    row duals, breaks ties only): exactly ``m`` BASIC, the rest at their
+    nearest bound (FIXED for a fixed variable, FREE for a free one).  The
+    objective is read as given (minimised)."""
+    params = params or SimplexParams()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if x.shape != (prob.n,):
+        raise ValueError(f"x has {x.size} entries, the LP {prob.n} columns")
+    yy = None
+    if y is not None:
+        yy = np.ascontiguousarray(y, dtype=np.float64)
+        if yy.shape != (prob.m,):
+            raise ValueError(f"y has {yy.size} entries, the LP {prob.m} rows")
+    h = host_lp(prob)
+    st = np.zeros(prob.n + prob.m, dtype=np.uint8)
+    placed = C.c_int32()
+    _lib.call("sovlp_basis_from_point", h.ref(), x.ctypes.data,
+              None if yy is None else yy.ctypes.data, float(params.pivot_tol),
+              float(tol_fixed), st.ctypes.data, C.byref(placed))
+    return st
+
+
+def crossover(prob: Problem, x, y=None, params=None) -> Solution:
+    """Cross over from the point ``x`` (duals ``y``, both in the caller's sense
+    and units, as ``solve_pdlp`` returns them) to an optimal basic solution:
+    the identified basis warm-starts the simplex, whose result (x, duals,
+    reduced costs, ``basis_status``) is exact to its tolerances."""
+    params = params or SimplexParams()
+    t0 = time.perf_counter()
+    # the basis is combinatorial; only the tie-break reads the sign of the
+    # reduced costs, which maximisation flips
+    work, yy = prob, y
+    if prob.sense == ObjSense.MAXIMISE:
+        work = prob.copy()
+        work.c = -work.c
+        work.sense = ObjSense.MINIMISE
+        if y is not None:
+            yy = -np.asarray(y, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if x.shape != (prob.n,):
+        raise ValueError(f"x has {x.size} entries, the LP {prob.n} columns")
+    if yy is not None:
+        yy = np.ascontiguousarray(yy, dtype=np.float64)
+        if yy.shape != (prob.m,):
+            raise ValueError(f"y has {yy.size} entries, the LP {prob.m} rows")
+    if params.sensitivity:
+        raise NotImplementedError("ranging (sensitivity=True) is not provided")
+    h = host_lp(work)
+    m, n = prob.m, prob.n
+    xo, y_out, d = np.zeros(n), np.zeros(m), np.zeros(n)
+    st = np.zeros(n + m, dtype=np.uint8)
+    res = _lib.sovlp_simplex_result()
+    cp = params.to_c()
+    _lib.call("sovlp_crossover", h.ref(), C.byref(cp), x.ctypes.data,
+              None if yy is None else yy.ctypes.data, xo.ctypes.data, y_out.ctypes.data,
+              d.ctypes.data, st.ctypes.data, C.byref(res))
+    return result_solution(prob, res, xo, y_out, d, st, t0, warm=True,
+                           extra={"crossover": True, "superbasics": int(res.superbasics),
+                                  "crossover_time": time.perf_counter() - t0})

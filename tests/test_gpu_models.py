@@ -64,6 +64,6 @@ def test_unprovided_methods_raise():
     from sovlp.solve import solve
     from sovlp.workloads import small_lp
     with pytest.raises(NotImplementedError):
-        solve(small_lp(0).problem, method="simplex")
+        solve(small_lp(0).problem, method="ipm")
     with pytest.raises(NotImplementedError):
-        solve(small_lp(0).problem, method="pdlp", crossover=True)
+        solve(small_lp(0).problem, method="pdlp", crossover=True, sensitivity=True)
