@@ -609,6 +609,23 @@ int sovlp_crossover(const sovlp_host_lp *lp, const sovlp_simplex_params *params,
                     const double *x0, const double *y0, double *x, double *y, double *d,
                     uint8_t *status, sovlp_simplex_result *result);
 
+/* Ranging of an optimal basis (the objective minimised, as given): the
+ * basis the n + m statuses name is factorised, its point, duals y = B^-T c_B
+ * and reduced costs recomputed, then
+ *   value (n)        the basis's structural values;
+ *   cost_lo/hi (n)   the interval of each c_j over which the basis stays
+ *                    optimal: a nonbasic cost moves its own reduced cost, a
+ *                    basic one every nonbasic reduced cost through its
+ *                    tableau row (one BTRAN per basic structural);
+ *   rhs, rhs_lo/hi (m) each row's active bound and the interval over which
+ *                    the basis stays feasible (one FTRAN per binding row;
+ *                    a non-binding row's bound may travel to its activity);
+ *   activity, dual (m), binding (m: 1 when the row's logical is nonbasic).
+ * Replaces compute_sensitivity (src/sovopt/lp/sensitivity.py:154-332). */
+int sovlp_simplex_ranging(const sovlp_host_lp *lp, const uint8_t *status, double *value,
+                          double *cost_lo, double *cost_hi, double *rhs, double *rhs_lo,
+                          double *rhs_hi, double *activity, double *dual, uint8_t *binding);
+
 /* Factorises the basis a status vector names (exactly m BASIC) and reports
  * how many of its columns had to be replaced by logicals (0: nonsingular).
  * The check of the reference's Basis.factorize / n_repairs

@@ -835,6 +835,112 @@ int sovlp_crossover(const sovlp_host_lp* lp, const sovlp_simplex_params* params,
   return 0;
 }
 
+int sovlp_simplex_ranging(const sovlp_host_lp* lp, const uint8_t* status, double* value,
+                          double* cost_lo, double* cost_hi, double* rhs, double* rhs_lo,
+                          double* rhs_hi, double* activity, double* dual, uint8_t* binding) {
+  Lp L;
+  if (int rc = load(lp, L)) return rc;
+  if (!status || !value || !cost_lo || !cost_hi || !rhs || !rhs_lo || !rhs_hi || !activity ||
+      !dual || !binding)
+    return fail(1, "null argument");
+  const int m = L.m, n = L.n, N = L.N;
+  int nb = 0;
+  for (int k = 0; k < N; ++k) {
+    if (status[k] > kFixed) return fail(1, "basis status out of range");
+    nb += status[k] == kBasic;
+  }
+  if (nb != m) return fail(1, "the status vector must hold exactly m basic entries");
+  sovlp_simplex_params P{};
+  P.feas_tol = 1e-9;
+  P.opt_tol = 1e-9;
+  P.pivot_tol = 1e-7;
+  P.refactor_freq = 1 << 30;
+  Simplex S(L, P);
+  start_from(S, status);
+  S.refactor();
+  if (S.repairs) return fail(1, "the basis is singular");
+  S.refresh();                            // z: nonbasics from statuses, x_B = -B^-1 N x_N
+  S.duals(false);                         // y = B^-T c_B
+  const std::vector<double> y(S.w1);
+  std::vector<double> d(N);
+  for (int k = 0; k < N; ++k) d[k] = L.cost[k] - L.dot(y.data(), k);
+  const std::vector<uint8_t>& st = S.st;
+  // cost ranging (sensitivity.py:154-185, 236-255): a nonbasic cost moves its
+  // own reduced cost one for one; a basic one moves every nonbasic reduced
+  // cost by -delta alpha_r[k], alpha_r the tableau row of its position
+  std::vector<double> rho, tmp;
+  for (int j = 0; j < n; ++j) {
+    value[j] = S.x[j];
+    cost_lo[j] = -INFINITY;
+    cost_hi[j] = INFINITY;
+    if (st[j] == kLower) cost_lo[j] = L.cost[j] - d[j];
+    else if (st[j] == kUpper) cost_hi[j] = L.cost[j] - d[j];
+  }
+  for (int r = 0; r < m; ++r) {
+    const int j = S.head[r];
+    if (j >= n) continue;                 // a logical carries no cost
+    rho.assign(m, 0.0);
+    rho[r] = 1.0;
+    S.F.btran(rho, tmp);
+    double lo = -INFINITY, hi = INFINITY;
+    for (int k = 0; k < N; ++k) {
+      const uint8_t s = st[k];
+      if (s == kBasic || s == kFixed) continue;
+      const double a = L.dot(rho.data(), k);
+      if (std::fabs(a) < 1e-12) continue;
+      const double ratio = d[k] / a;
+      const bool keep_nonneg = s == kLower || (s == kFree && d[k] >= 0.0);
+      if (keep_nonneg == (a > 0.0)) hi = std::min(hi, ratio);
+      else lo = std::max(lo, ratio);
+    }
+    cost_lo[j] = L.cost[j] + lo;
+    cost_hi[j] = L.cost[j] + hi;
+  }
+  // right-hand-side ranging (sensitivity.py:187-212, 257-279): a binding row's
+  // bound moves its logical, and with it x_B along beta = B^-1 e_i
+  std::vector<double> beta;
+  for (int i = 0; i < m; ++i) {
+    const int k = n + i;
+    const double lo_i = L.lo[k], hi_i = L.up[k];
+    activity[i] = S.x[k];
+    dual[i] = y[i];
+    if (st[k] == kBasic) {
+      binding[i] = 0;
+      rhs[i] = lo_i > -kInf ? lo_i : hi_i;
+      rhs_lo[i] = -INFINITY;
+      rhs_hi[i] = S.x[k];
+      if (hi_i < kInf && lo_i <= -kInf) {
+        rhs_lo[i] = S.x[k];
+        rhs_hi[i] = INFINITY;
+      }
+      continue;
+    }
+    binding[i] = 1;
+    const double b = (st[k] == kLower || st[k] == kFixed) ? lo_i : hi_i;
+    rhs[i] = b;
+    beta.assign(m, 0.0);
+    beta[i] = 1.0;
+    S.F.ftran(beta, tmp);
+    double lo = -INFINITY, hi = INFINITY;
+    for (int r = 0; r < m; ++r) {
+      const double a = beta[r];
+      if (std::fabs(a) < 1e-12) continue;
+      const int q = S.head[r];
+      const double zb = S.x[q], l = L.lo[q], u = L.up[q];
+      if (a > 0.0) {
+        if (l > -kInf) lo = std::max(lo, (l - zb) / a);
+        if (u < kInf) hi = std::min(hi, (u - zb) / a);
+      } else {
+        if (u < kInf) lo = std::max(lo, (u - zb) / a);
+        if (l > -kInf) hi = std::min(hi, (l - zb) / a);
+      }
+    }
+    rhs_lo[i] = b + lo;
+    rhs_hi[i] = b + hi;
+  }
+  return 0;
+}
+
 int sovlp_basis_repairs(const sovlp_host_lp* lp, const uint8_t* status, int32_t* repairs) {
   Lp L;
   if (int rc = load(lp, L)) return rc;

@@ -176,6 +176,8 @@ _SIGS = {
     "sovlp_basis_from_point": (C.c_int, [C.POINTER(sovlp_host_lp), P, P, F64, F64, P,
                                          C.POINTER(I32)]),
     "sovlp_basis_repairs": (C.c_int, [C.POINTER(sovlp_host_lp), P, C.POINTER(I32)]),
+    "sovlp_simplex_ranging": (C.c_int, [C.POINTER(sovlp_host_lp), P, P, P, P, P, P, P, P, P,
+                                        P]),
     "sovlp_crossover": (C.c_int, [C.POINTER(sovlp_host_lp), C.POINTER(sovlp_simplex_params),
                                   P, P, P, P, P, P, C.POINTER(sovlp_simplex_result)]),
 }

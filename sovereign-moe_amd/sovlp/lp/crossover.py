@@ -86,8 +86,6 @@ def crossover(prob: Problem, x, y=None, params=None) -> Solution:
         yy = np.ascontiguousarray(yy, dtype=np.float64)
         if yy.shape != (prob.m,):
             raise ValueError(f"y has {yy.size} entries, the LP {prob.m} rows")
-    if params.sensitivity:
-        raise NotImplementedError("ranging (sensitivity=True) is not provided")
     h = host_lp(work)
     m, n = prob.m, prob.n
     xo, y_out, d = np.zeros(n), np.zeros(m), np.zeros(n)
@@ -99,4 +97,5 @@ def crossover(prob: Problem, x, y=None, params=None) -> Solution:
               d.ctypes.data, st.ctypes.data, C.byref(res))
     return result_solution(prob, res, xo, y_out, d, st, t0, warm=True,
                            extra={"crossover": True, "superbasics": int(res.superbasics),
-                                  "crossover_time": time.perf_counter() - t0})
+                                  "crossover_time": time.perf_counter() - t0},
+                           sensitivity=params.sensitivity)

@@ -9,9 +9,10 @@ revised simplex over ``[A | -I]``: phase 1 on the sum of infeasibilities,
 Dantzig pricing, a Harris two-pass ratio test with bound flips, a random cost
 perturbation after ``perturb_after`` degenerate pivots (removed before the end),
 a dense LU of the basis refactorised every ``refactor_freq`` pivots with
-product-form updates between.  Dense factors limit it to 8192 rows.  The
-reference's dual simplex, node kernel, Forrest-Tomlin updates, scaling and
-ranging are not provided (``sensitivity=True`` raises).
+product-form updates between.  Dense factors limit it to 8192 rows.
+``sensitivity=True`` adds the ranging of the optimal basis
+(``sovlp.lp.sensitivity``).  The reference's dual simplex, node kernel,
+Forrest-Tomlin updates and scaling are not provided.
 """
 
 from __future__ import annotations
@@ -55,8 +56,6 @@ def solve_simplex(prob: Problem, params: SimplexParams | None = None,
     statuses with exactly ``m`` BASIC) or from the all-logical basis."""
     params = params or SimplexParams()
     t0 = time.perf_counter()
-    if params.sensitivity:
-        raise NotImplementedError("ranging (sensitivity=True) is not provided")
     if prob.Q is not None:
         raise NotImplementedError("solve_simplex optimises a linear objective; this model "
                                   "has a quadratic term")
@@ -78,13 +77,15 @@ def solve_simplex(prob: Problem, params: SimplexParams | None = None,
     _lib.call("sovlp_simplex_solve", h.ref(), C.byref(cp),
               None if warm is None else warm.ctypes.data, x.ctypes.data, y.ctypes.data,
               d.ctypes.data, st.ctypes.data, C.byref(res))
-    return result_solution(prob, res, x, y, d, st, t0, warm is not None)
+    return result_solution(prob, res, x, y, d, st, t0, warm is not None,
+                           sensitivity=params.sensitivity)
 
 
 def result_solution(prob: Problem, res, x, y, d, st, t0: float, warm: bool,
-                    extra: dict | None = None) -> Solution:
+                    extra: dict | None = None, sensitivity: bool = False) -> Solution:
     """The ``Solution`` of a native simplex result (duals in the caller's
-    sense: the native code minimises, so a maximisation flips y and d)."""
+    sense: the native code minimises, so a maximisation flips y and d), with
+    the ranging of an optimal basis when ``sensitivity``."""
     status = Status(int(res.status))
     if prob.sense == ObjSense.MAXIMISE:
         y, d = -y, -d
@@ -97,4 +98,7 @@ def result_solution(prob: Problem, res, x, y, d, st, t0: float, warm: bool,
                 "repairs": int(res.repairs), "primal_infeasibility": res.primal_infeasibility,
                 "dual_infeasibility": res.dual_infeasibility, "native_seconds": res.seconds,
                 "warm": bool(warm), **(extra or {})}
+    if sensitivity and status == Status.OPTIMAL:
+        from .sensitivity import compute_sensitivity
+        sol.sensitivity = compute_sensitivity(prob, st)
     return sol.drop_objective_if_unsolved()

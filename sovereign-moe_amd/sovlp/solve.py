@@ -19,12 +19,14 @@ __all__ = ["solve", "solve_file"]
 
 def solve(prob: Problem, method: str = "auto", device: str = "auto",
           time_limit: float = 300.0, gap: float = 1e-4, tol: float = 1e-8,
-          verbose: bool = False, crossover: bool = False, **options) -> Solution:
+          verbose: bool = False, crossover: bool = False, sensitivity: bool = False,
+          **options) -> Solution:
     """Solve an LP with PDLP on the GPU (``crossover``: then cross over to an
     optimal basis), or with the host simplex (``method="simplex"``).  ``gap``
     is accepted for the reference's signature (it applies to branch-and-bound,
-    not to an LP); the reference's other methods and options (``ipm``,
-    ``sensitivity``, ``presolve``) are not provided and raise."""
+    not to an LP); ``sensitivity`` adds the ranging of the simplex's or the
+    crossover's optimal basis.  The reference's other methods and options
+    (``ipm``, ``presolve``) are not provided and raise."""
     from .lp.pdlp import PDLPParams, solve_pdlp
     from .lp.simplex import SimplexParams, solve_simplex
     if method not in ("auto", "pdlp", "simplex"):
@@ -33,7 +35,7 @@ def solve(prob: Problem, method: str = "auto", device: str = "auto",
     if unsupported:
         raise NotImplementedError(f"options not provided: {', '.join(sorted(unsupported))}")
     sp = SimplexParams(time_limit=time_limit, feas_tol=max(tol, 1e-9),
-                       opt_tol=max(tol, 1e-9), verbose=verbose)
+                       opt_tol=max(tol, 1e-9), sensitivity=sensitivity, verbose=verbose)
     if method == "simplex":
         return solve_simplex(prob, sp)
     point = solve_pdlp(prob, PDLPParams(device=device, eps_abs=tol, eps_rel=tol,

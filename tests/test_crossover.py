@@ -270,8 +270,6 @@ def test_arguments_are_checked():
         solve_simplex(p, warm_basis=np.zeros(3, dtype=np.uint8))
     with pytest.raises(ValueError, match="x has"):
         basis_from_point(p, np.zeros(p.n + 1))
-    with pytest.raises(NotImplementedError, match="ranging"):
-        solve_simplex(p, SimplexParams(sensitivity=True))
     B = Basis(p)
     B.status[:] = AT_LOWER
     with pytest.raises(ValueError, match="exactly m"):

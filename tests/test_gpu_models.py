@@ -66,4 +66,4 @@ def test_unprovided_methods_raise():
     with pytest.raises(NotImplementedError):
         solve(small_lp(0).problem, method="ipm")
     with pytest.raises(NotImplementedError):
-        solve(small_lp(0).problem, method="pdlp", crossover=True, sensitivity=True)
+        solve(small_lp(0).problem, method="pdlp", presolve=True)
