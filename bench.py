@@ -19,6 +19,8 @@ Beside the headline number the JSON line carries:
   engine_default_stalls  the batch again with the engine's default stall
                 lengths (4096 before a refined point meets the criterion):
                 more LPs finish, in more time
+  fp64_batch    the batch in FP64 on the same batched engine (one timed
+                step): what batching alone buys, and the ratio to it
   roofline      algorithmic bytes of the step kernel (phase counts x the byte
                 model of each phase) / its average launch time (HIP events on
                 the launch stream), against 8 TB/s; `traffic` the HBM bytes per
@@ -92,6 +94,9 @@ def parse():
     ap.add_argument("--robust-sample", type=int, default=1,
                     help="FP32: steps also run with the engine's default stall lengths, "
                          "reported beside the headline (0: skip)")
+    ap.add_argument("--fp64-batch-steps", type=int, default=1,
+                    help="FP32: timed steps of the same batch in FP64, reported beside the "
+                         "headline (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="LPs solved one at a time for the FP64 single-LP rate (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -142,6 +147,35 @@ def refinement_config(par, precision: int):
             "stall_open": par.refine_stall_open or 4096, "target": par.refine_tol}
 
 
+def fp64_batch_rate(problems, tol: float, steps: int) -> dict:
+    """The same batch in FP64 on the same batched engine (256-thread
+    workgroups, no refinement): the batching without the mixed precision,
+    so the headline's gain splits into what batching and what FP32 +
+    refinement buy.  One untimed step, then ``steps`` timed ones."""
+    import torch
+    from sovlp.lp.batch import BatchEngine
+    from sovlp.lp.pdlp import PDLPParams
+    par = PDLPParams(eps_abs=tol, eps_rel=tol, feas_cap=tol, device="gpu")
+    eng = BatchEngine(problems, par, precision=64, block=256)
+    try:
+        eng.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.run()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        sols, lat = eng.solutions(keep_outputs=False)
+    finally:
+        eng.close()
+    import numpy as np
+    opt = sum(1 for s in sols if int(s.status) == 0)
+    return {"value": opt * steps / wall, "unit": "LPs/s", "optimal": opt,
+            "ms_per_step": wall / steps * 1e3, "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)), "max_iterations":
+            int(max(s.iterations for s in sols))}
+
+
 def robust_rate(problems, par, tuner, a) -> dict:
     """The same batch with the engine's default stall lengths (rounds before a
     refined point meets the criterion may run 4x longer): more LPs finish, in
@@ -185,7 +219,8 @@ def bnr_rate(K: int = 128, iters: int = 60, reps: int = 10) -> dict:
         js = rng.choice(p.n, 20, replace=False)
         U[js, q] = np.minimum(U[js, q], rng.uniform(0.0, 5.0, 20))
     eng = BNREngine(p, config=BNRConfig(batch=K, iters=iters, textbook_duals=True))
-    eng.solve_batch(L, U)
+    for _ in range(2):          # the second call captures the launches as a graph
+        eng.solve_batch(L, U)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(reps):
@@ -737,6 +772,12 @@ def main():
         log(f"plain fp64 sample: {pl} ({time.perf_counter() - t:.1f} s)")
         out["plain_fp64"] = pl
         out["speedup_vs_plain_fp64"] = out["value"] / pl["value"]
+    if a.precision == 32 and a.fp64_batch_steps > 0:
+        t = time.perf_counter()
+        fb64 = fp64_batch_rate(problems, a.tol, a.fp64_batch_steps)
+        log(f"fp64 batch: {fb64} ({time.perf_counter() - t:.1f} s)")
+        out["fp64_batch"] = fb64
+        out["speedup_vs_fp64_batch"] = out["value"] / fb64["value"]
     if a.cpu_seconds > 0:
         cb = cpu_baseline(problems, a.tol, a.cpu_seconds)
         log(f"cpu baseline: {cb}")

@@ -516,6 +516,93 @@ int sovlp_bnr_workspace_bytes(int m, int n, int K, int64_t* bytes) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The launches of one node-relaxation call (start, `iters` iterations of five
+// launches, the bound), enqueued on `st`.
+int bnr_enqueue(const BnrPtrs& P, double* part_p, double* part_d, double* part_b, int gm,
+                int gn, int iters, double growth, double damp, hipStream_t st) {
+  const int m = P.m, n = P.n, K = P.K;
+  const int64_t nk = (int64_t)n * K;
+  const int gk = (K + 63) / 64;
+  bnr_zero_kernel<<<gk, 64, 0, st>>>(P.empty, K);
+  HIPCHK(hipGetLastError());
+  bnr_start_kernel<<<gn, kBnrBlock, 0, st>>>(P);
+  HIPCHK(hipGetLastError());
+  if (m > 0) bnr_spmm_kernel<<<gm, kBnrBlock, 0, st>>>(m, K, P.KP, P.RB, P.rp, P.ri, P.rx, P.X, P.AX);
+  bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, P.KP, P.RB, P.cp, P.ci, P.cx, P.Y, P.AtY);
+  HIPCHK(hipGetLastError());
+  if (m == 0) HIPCHK(hipMemsetAsync(P.AtdY, 0, sizeof(double) * nk, st));   // A^T dY = 0
+  BnrPtrs Pp = P, Pd = P, Pb = P;
+  Pp.part = part_p;
+  Pd.part = part_d;
+  Pb.part = part_b;
+  for (int it = 0; it < iters; ++it) {
+    // from the second iteration on, the previous iteration's update runs in
+    // the primal launch (its row part in gm extra blocks)
+    const int pending = it > 0 ? 1 : 0;
+    bnr_primal_kernel<<<gn + (pending && m > 0 ? gm : 0), kBnrBlock, 0, st>>>(Pp, pending, gn);
+    if (m > 0) {
+      bnr_spmm_kernel<<<gm, kBnrBlock, 0, st>>>(m, K, P.KP, P.RB, P.rp, P.ri, P.rx, P.dX, P.AdX);
+      bnr_dual_kernel<<<gm, kBnrBlock, 0, st>>>(Pd);
+      bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, P.KP, P.RB, P.cp, P.ci, P.cx, P.dY, P.AtdY);
+    }
+    const double t = (double)(it + 2);
+    bnr_decide_kernel<<<K, kBnrLanes, 0, st>>>(P, part_p, gn, part_d, m > 0 ? gm : 0,
+                                               1.0 - std::pow(t, -growth),
+                                               1.0 + std::pow(t, -damp));
+    HIPCHK(hipGetLastError());
+  }
+  if (iters > 0) bnr_update_kernel<<<gn + (m > 0 ? gm : 0), kBnrBlock, 0, st>>>(P, gn);
+  // the bound from the final Y: A^T Y into AtdY, then the terms
+  if (m > 0) {
+    bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, P.KP, P.RB, P.cp, P.ci, P.cx, P.Y, P.AtdY);
+  } else {
+    HIPCHK(hipMemsetAsync(P.AtdY, 0, sizeof(double) * nk, st));
+  }
+  bnr_bound_terms_kernel<<<(m > 0 ? gm : 0) + gn, kBnrBlock, 0, st>>>(Pb, m > 0 ? gm : 0);
+  bnr_bound_final_kernel<<<K, kBnrLanes, 0, st>>>(Pb, m > 0 ? gm : 0, gn);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// A call's launches depend only on the call's arguments, so a repeated call
+// (a tree visiting batch after batch through the same buffers) replays a
+// HIP graph of them: captured on the second call with the same arguments,
+// replayed from the third.  At most kBnrGraphs argument sets per process
+// are kept; beyond that, and for LPs without rows (whose zeroing is a
+// memset), calls launch directly.
+constexpr int kBnrGraphs = 16;
+struct BnrGraph {
+  std::vector<uint64_t> key;
+  int seen = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+std::mutex g_bnr_mu;
+std::vector<BnrGraph> g_bnr_graphs;
+std::vector<hipStream_t> g_bnr_capture;   // per device: the capture stream
+
+std::vector<uint64_t> bnr_key(const sovlp_bnr_batch* b, const void* ws, int dev) {
+  auto u = [](const void* p) { return (uint64_t)reinterpret_cast<uintptr_t>(p); };
+  auto f = [](double v) {
+    uint64_t r;
+    std::memcpy(&r, &v, sizeof r);
+    return r;
+  };
+  return {(uint64_t)dev, (uint64_t)b->m, (uint64_t)b->n, (uint64_t)b->K, (uint64_t)b->iters,
+          u(b->rp), u(b->cp), u(b->ri), u(b->ci), u(b->rx), u(b->cx), u(b->c), u(b->rl),
+          u(b->ru), u(b->L), u(b->U), u(b->X), u(b->Y), u(b->eta), u(b->omega),
+          f(b->step_growth), f(b->step_damp), u(b->bounds), u(b->empty), f(b->dual_sign),
+          u(ws)};
+}
+
+}  // namespace
+
+extern "C" {
+
 int sovlp_bnr_solve_batch(const sovlp_bnr_batch* b, void* workspace, int64_t workspace_bytes,
                           sovlp_stream_t stream) {
   if (!b || !workspace) return fail(1, "null argument");
@@ -551,45 +638,48 @@ int sovlp_bnr_solve_batch(const sovlp_bnr_batch* b, void* workspace, int64_t wor
   P.empty = b->empty;
   P.bounds = b->bounds;
   hipStream_t st = S(stream);
-  const int gk = (K + 63) / 64;
-  bnr_zero_kernel<<<gk, 64, 0, st>>>(P.empty, K);
-  HIPCHK(hipGetLastError());
-  bnr_start_kernel<<<gn, kBnrBlock, 0, st>>>(P);
-  HIPCHK(hipGetLastError());
-  if (m > 0) bnr_spmm_kernel<<<gm, kBnrBlock, 0, st>>>(m, K, KP, RB, P.rp, P.ri, P.rx, P.X, P.AX);
-  bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, KP, RB, P.cp, P.ci, P.cx, P.Y, P.AtY);
-  HIPCHK(hipGetLastError());
-  if (m == 0) HIPCHK(hipMemsetAsync(P.AtdY, 0, sizeof(double) * nk, st));   // A^T dY = 0
-  BnrPtrs Pp = P, Pd = P, Pb = P;
-  Pp.part = part_p;
-  Pd.part = part_d;
-  Pb.part = part_b;
-  for (int it = 0; it < b->iters; ++it) {
-    // from the second iteration on, the previous iteration's update runs in
-    // the primal launch (its row part in gm extra blocks)
-    const int pending = it > 0 ? 1 : 0;
-    bnr_primal_kernel<<<gn + (pending && m > 0 ? gm : 0), kBnrBlock, 0, st>>>(Pp, pending, gn);
-    if (m > 0) {
-      bnr_spmm_kernel<<<gm, kBnrBlock, 0, st>>>(m, K, KP, RB, P.rp, P.ri, P.rx, P.dX, P.AdX);
-      bnr_dual_kernel<<<gm, kBnrBlock, 0, st>>>(Pd);
-      bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, KP, RB, P.cp, P.ci, P.cx, P.dY, P.AtdY);
+  if (m == 0)
+    return bnr_enqueue(P, part_p, part_d, part_b, gm, gn, b->iters, b->step_growth,
+                       b->step_damp, st);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  const std::vector<uint64_t> key = bnr_key(b, workspace, dev);
+  std::lock_guard<std::mutex> lock(g_bnr_mu);
+  BnrGraph* g = nullptr;
+  for (BnrGraph& e : g_bnr_graphs)
+    if (e.key == key) g = &e;
+  if (!g && (int)g_bnr_graphs.size() < kBnrGraphs) {
+    g_bnr_graphs.push_back(BnrGraph{key});
+    g = &g_bnr_graphs.back();
+  }
+  if (!g || ++g->seen < 2)
+    return bnr_enqueue(P, part_p, part_d, part_b, gm, gn, b->iters, b->step_growth,
+                       b->step_damp, st);
+  if (!g->exec) {
+    if ((int)g_bnr_capture.size() <= dev) g_bnr_capture.resize(dev + 1, nullptr);
+    if (!g_bnr_capture[dev])
+      HIPCHK(hipStreamCreateWithFlags(&g_bnr_capture[dev], hipStreamNonBlocking));
+    hipStream_t cs = g_bnr_capture[dev];
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    const int rc = bnr_enqueue(P, part_p, part_d, part_b, gm, gn, b->iters, b->step_growth,
+                               b->step_damp, cs);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
     }
-    const double t = (double)(it + 2);
-    bnr_decide_kernel<<<K, kBnrLanes, 0, st>>>(P, part_p, gn, part_d, m > 0 ? gm : 0,
-                                         1.0 - std::pow(t, -b->step_growth),
-                                         1.0 + std::pow(t, -b->step_damp));
-    HIPCHK(hipGetLastError());
+    if (ec != hipSuccess) return fail(2, std::string("bnr graph capture: ") + hipGetErrorString(ec));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(2, std::string("bnr graph instantiate: ") + hipGetErrorString(ei));
+    }
+    g->graph = graph;
+    g->exec = exec;
   }
-  if (b->iters > 0) bnr_update_kernel<<<gn + (m > 0 ? gm : 0), kBnrBlock, 0, st>>>(P, gn);
-  // the bound from the final Y: A^T Y into AtdY, then the terms
-  if (m > 0) {
-    bnr_spmm_kernel<<<gn, kBnrBlock, 0, st>>>(n, K, KP, RB, P.cp, P.ci, P.cx, P.Y, P.AtdY);
-  } else {
-    HIPCHK(hipMemsetAsync(P.AtdY, 0, sizeof(double) * nk, st));
-  }
-  bnr_bound_terms_kernel<<<(m > 0 ? gm : 0) + gn, kBnrBlock, 0, st>>>(Pb, m > 0 ? gm : 0);
-  bnr_bound_final_kernel<<<K, kBnrLanes, 0, st>>>(Pb, m > 0 ? gm : 0, gn);
-  HIPCHK(hipGetLastError());
+  HIPCHK(hipGraphLaunch(g->exec, st));
   return 0;
 }
 

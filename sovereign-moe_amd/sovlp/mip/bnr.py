@@ -97,6 +97,33 @@ class BNREngine:
         if K > self.K:
             self._alloc(min(256, max(K, self.K * 2)))
 
+    def _buffers(self, K: int) -> dict:
+        """Device buffers and the ``sovlp_bnr_batch`` descriptor of a K-node call."""
+        import torch
+        calls = self.__dict__.setdefault("_calls", {})
+        if K in calls:
+            return calls[K]
+        m, n = self.m, self.n
+        f = lambda *s: torch.zeros(*s, dtype=torch.float64, device="cuda")   # noqa: E731
+        c = dict(L=f(n, K), U=f(n, K), X=f(n, K), Y=f(max(m, 1), K), eta=f(K), omega=f(K),
+                 bounds=f(K), empty=torch.zeros(K, dtype=torch.int32, device="cuda"))
+        nb = C.c_int64()
+        _lib.call("sovlp_bnr_workspace_bytes", m, n, K, C.byref(nb))
+        c["ws"] = torch.empty(int(nb.value), dtype=torch.uint8, device="cuda")
+        calls[K] = c
+        return c
+
+    def _descriptor(self, K: int, c: dict):
+        t = self.t
+        return _lib.sovlp_bnr_batch(
+            self.m, self.n, K, int(self.cfg.iters), t["rp"].data_ptr(), t["cp"].data_ptr(),
+            t["ri"].data_ptr(), t["ci"].data_ptr(), t["rx"].data_ptr(), t["cx"].data_ptr(),
+            t["c"].data_ptr(), t["rl"].data_ptr(), t["ru"].data_ptr(), c["L"].data_ptr(),
+            c["U"].data_ptr(), c["X"].data_ptr(), c["Y"].data_ptr(), c["eta"].data_ptr(),
+            c["omega"].data_ptr(), float(self.cfg.step_growth), float(self.cfg.step_damp),
+            c["bounds"].data_ptr(), c["empty"].data_ptr(),
+            -1.0 if self.cfg.textbook_duals else 1.0)
+
     def solve_batch(self, col_lb, col_ub, warm=None) -> BatchResult:
         """Advance ``K`` nodes (``col_lb``, ``col_ub``: ``(n, K)``) and bound them.
         ``warm``: an optional ``(X, Y)`` from an earlier visit."""
@@ -109,33 +136,26 @@ class BNREngine:
         if not 1 <= K <= 256:
             raise ValueError("K must be in [1, 256]")
         self._ensure(K)
-        m, n = self.m, self.n
-        L = torch.from_numpy(np.ascontiguousarray(col_lb)).to("cuda")
-        U = torch.from_numpy(np.ascontiguousarray(col_ub)).to("cuda")
-        X = torch.zeros(n, K, dtype=torch.float64, device="cuda")
-        Y = torch.zeros(max(m, 1), K, dtype=torch.float64, device="cuda")
+        m = self.m
+        # the call's device buffers persist per batch size, so repeated calls
+        # pass the library the same arguments (it then replays them as a graph)
+        c = self._buffers(K)
+        L, U, X, Y = c["L"], c["U"], c["X"], c["Y"]
+        eta, omega, bounds, empty, ws = c["eta"], c["omega"], c["bounds"], c["empty"], c["ws"]
+        L.copy_(torch.from_numpy(np.ascontiguousarray(col_lb)))
+        U.copy_(torch.from_numpy(np.ascontiguousarray(col_ub)))
         if warm is not None:
             X.copy_(torch.from_numpy(np.ascontiguousarray(warm[0], dtype=np.float64)))
             if m > 0:
                 Y.copy_(torch.from_numpy(np.ascontiguousarray(warm[1], dtype=np.float64)))
-        eta = torch.from_numpy(self.eta[:K].copy()).to("cuda")
-        omega = torch.from_numpy(self.omega[:K].copy()).to("cuda")
-        bounds = torch.empty(K, dtype=torch.float64, device="cuda")
-        empty = torch.empty(K, dtype=torch.int32, device="cuda")
-        nb = C.c_int64()
-        _lib.call("sovlp_bnr_workspace_bytes", m, n, K, C.byref(nb))
-        ws = torch.empty(int(nb.value), dtype=torch.uint8, device="cuda")
-        t = self.t
-        b = _lib.sovlp_bnr_batch(m, n, K, int(self.cfg.iters), t["rp"].data_ptr(),
-                                 t["cp"].data_ptr(), t["ri"].data_ptr(), t["ci"].data_ptr(),
-                                 t["rx"].data_ptr(), t["cx"].data_ptr(), t["c"].data_ptr(),
-                                 t["rl"].data_ptr(), t["ru"].data_ptr(), L.data_ptr(),
-                                 U.data_ptr(), X.data_ptr(), Y.data_ptr(), eta.data_ptr(),
-                                 omega.data_ptr(), float(self.cfg.step_growth),
-                                 float(self.cfg.step_damp), bounds.data_ptr(),
-                                 empty.data_ptr(), -1.0 if self.cfg.textbook_duals else 1.0)
+        else:
+            X.zero_()
+            Y.zero_()
+        eta.copy_(torch.from_numpy(self.eta[:K].copy()))
+        omega.copy_(torch.from_numpy(self.omega[:K].copy()))
         # the stream the uploads above were queued on
-        _lib.call("sovlp_bnr_solve_batch", C.byref(b), ws.data_ptr(), ws.numel(),
+        desc = self._descriptor(K, c)
+        _lib.call("sovlp_bnr_solve_batch", C.byref(desc), ws.data_ptr(), ws.numel(),
                   torch.cuda.current_stream().cuda_stream)
         self.X[:, :K] = X
         self.Y[:, :K] = Y[: self.Y.shape[0]]

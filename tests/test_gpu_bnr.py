@@ -114,3 +114,34 @@ def test_calls_under_another_stream_match_the_default_stream():
     torch.cuda.synchronize()
     assert np.array_equal(a.bounds, b.bounds) and np.array_equal(a.x, b.x)
     assert got == ref
+
+
+def test_repeated_calls_replay_the_same_launches():
+    """The library replays a repeated call as a HIP graph (captured on the
+    second call with the same arguments): a first (direct), second (captured)
+    and later (replayed) call on the same boxes give the same results bit for
+    bit, and a call with other boxes in between does not leak into them."""
+    from sovlp.mip.bnr import BNRConfig, BNREngine
+    k = known_optimum_lp(30, 60, 3, seed=6, m_eq=15)
+    p = k.problem
+    L, U = node_boxes(p, 16, seed=4)
+    L2, U2 = node_boxes(p, 16, seed=9)
+    print("[gpu] bnr: repeated calls", flush=True)
+    eng = BNREngine(p, config=BNRConfig(batch=16, iters=40, textbook_duals=True))
+    runs = []
+    for i in range(5):
+        # each visit from the same state: cold, the engine's step sizes reset
+        eng.eta[:] = 1.0
+        eng.omega[:] = 1.0
+        if i == 3:
+            other = eng.solve_batch(L2, U2)
+            eng.eta[:] = 1.0
+            eng.omega[:] = 1.0
+        r = eng.solve_batch(L, U)
+        runs.append((r.bounds.copy(), r.x.copy(), eng.eta[:16].copy()))
+    for b, x, e in runs[1:]:
+        assert np.array_equal(b, runs[0][0]) and np.array_equal(x, runs[0][1])
+        assert np.array_equal(e, runs[0][2])
+    assert not np.array_equal(other.bounds, runs[0][0])
+    fresh = BNREngine(p, config=BNRConfig(batch=16, iters=40, textbook_duals=True))
+    assert np.array_equal(fresh.solve_batch(L2, U2).bounds, other.bounds)
