@@ -89,7 +89,7 @@ namespace sovh {
 
 
 constexpr int kTnnzSmall = 512;   // 64-thread workgroups: 2 x 512 x 8 B = 8 KiB LDS
-constexpr int kPairsPerGraph = 64;     // W > 1: phase + decide launches per graph
+constexpr int kPairsPerGraph = 64;     // W > 1: step launches (one phase each) per graph
 
 // Workgroup sizes of the batched engine, and their tile nonzero caps (16 KiB
 // of LDS for 128 threads, 32 KiB for 256, 64 KiB for 512: the largest LPs
@@ -161,7 +161,7 @@ int pick_width(const sovlp_lp* lp, int team, int* W) {
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, pdlp_phase_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>, kBlock, 0));
+      &nb, pdlp_step_kernel<T, kBlock, sizeof(T) == 8 ? kTnnz64 : kTnnz32>, kBlock, 0));
   const long long tiles = (long long)lp->ntiles_a + lp->ntiles_t;
   const long long resident = (long long)std::max(nb, 1) * cus;
   long long w = std::max<long long>(1, tiles / 4);
@@ -177,7 +177,8 @@ size_t single_ws_T(const sovlp_lp* lp, long long hist_cap, int W) {
   cv.take<LpDev<T>>(1);
   cv.take<StepCtl>(1);
   cv.take<unsigned>(64);
-  cv.take<double>((size_t)W * kMaxRed);
+  cv.take<double>((size_t)2 * W * kMaxRed);
+  cv.take<LpState>(1);
   cv.take<int32_t>(W + 1);
   cv.take<int32_t>(W + 1);
   cv.take<long long>(kPhaseSlots);
@@ -198,7 +199,8 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   LpDev<T>* d_lp = cv.take<LpDev<T>>(1);
   StepCtl* ctl = cv.take<StepCtl>(1);
   unsigned* done = cv.take<unsigned>(64);
-  double* partials = cv.take<double>((size_t)W * kMaxRed);
+  double* partials = cv.take<double>((size_t)2 * W * kMaxRed);
+  LpState* st2 = cv.take<LpState>(1);
   int32_t* partA = cv.take<int32_t>(W + 1);
   int32_t* partT = cv.take<int32_t>(W + 1);
   long long* pcount = cv.take<long long>(kPhaseSlots);
@@ -207,6 +209,7 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   carve_lp<T>(cv, L, lp->m, lp->n, lp->nnz, hist_cap);
   fill_lp<T>(L, *lp);
   L.partials = partials;
+  L.st2 = st2;
   L.pcount = pcount;
   L.pticks = pticks;
   if (W > 1) {
@@ -261,7 +264,8 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
   h->ticks_set = c0.ticks;
   // record the launches of one replay into a graph on a private capture
   // stream: W == 1, one run launch (kPhasesPerLaunch phases); W > 1,
-  // kPairsPerGraph phase + decide pairs (one phase each)
+  // kPairsPerGraph step launches (one phase each, alternating state copies),
+  // then the settle launch that takes the last phase's transition
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
   constexpr int TN = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
@@ -269,10 +273,11 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
     launch_run<T>(kBlock, d_lp, 1, ctl, done, kPhasesPerLaunch, h->cap);
     h->phases_per_replay = kPhasesPerLaunch;
   } else {
-    for (int i = 0; i < kPairsPerGraph; ++i) {
-      pdlp_phase_kernel<T, kBlock, TN><<<W, kBlock, 0, h->cap>>>(d_lp);
-      pdlp_decide_kernel<T, kBlock><<<1, kBlock, 0, h->cap>>>(d_lp, W, ctl, done);
-    }
+    static_assert(kPairsPerGraph % 2 == 0, "the last step launch must write L.st");
+    for (int i = 0; i < kPairsPerGraph; ++i)
+      pdlp_step_kernel<T, kBlock, TN><<<W, kBlock, 0, h->cap>>>(d_lp, i & 1, ctl, done);
+    pdlp_settle_kernel<T, kBlock><<<1, kBlock, 0, h->cap>>>(d_lp, W, (kPairsPerGraph - 1) & 1,
+                                                            ctl, done);
     h->phases_per_replay = kPairsPerGraph;
   }
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));

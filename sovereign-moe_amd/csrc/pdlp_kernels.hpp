@@ -4,9 +4,11 @@
 //   pdlp_run_kernel      one workgroup per LP runs up to `max_phases` phases of
 //                        its LP back to back (state in LDS, workgroup barriers
 //                        between phases; no workgroup waits for another)
-//   pdlp_phase_kernel    W workgroups share one LP's phase, write partial sums
-//   pdlp_decide_kernel   one workgroup reduces the W partials (fixed order) and
-//                        takes the transition
+//   pdlp_step_kernel     W workgroups share one LP's phase: each first takes
+//                        the previous phase's transition from the W partial
+//                        sums (fixed order, the same in every workgroup), then
+//                        runs the phase and writes its partial sums
+//   pdlp_settle_kernel   one workgroup takes the last pending transition
 //   state_reset_kernel   puts a batch back at the start of its solves
 #pragma once
 #include "pdlp_solver.hpp"
@@ -33,6 +35,13 @@ __device__ __forceinline__ void init_part(int ph, double (&part)[kMaxRed]) {
 #pragma unroll
   for (int k = 0; k < kMaxRed; ++k) part[k] = 0.0;
   if (ph == PH_PC_APPLY) part[1] = -__builtin_inf();
+}
+
+// Whether a phase's transition reads its sums: the fused A phase reduces
+// nothing, and the fused B phase only ||dx||^2 for a step-size trial that
+// follows it (the transition reads it then only).
+__device__ __forceinline__ bool phase_reduces(int ph, int next_mode) {
+  return !(ph == PH_A_FUSED || (ph == PH_B_FUSED && next_mode != NEXT_TRIAL));
 }
 
 // Workgroup totals of a phase's partial sums: CHECK has NCHK values, every
@@ -132,8 +141,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
     // the fused A phase reduces nothing, and the fused B phase only ||dx||^2
     // for a step-size trial that follows it.  Read here, while no thread
     // writes the state: thread 0 changes it right after the phase.
-    const bool reduce =
-        !(ph == PH_A_FUSED || (ph == PH_B_FUSED && uniform(ss.next_mode) != NEXT_TRIAL));
+    const bool reduce = phase_reduces(ph, uniform(ss.next_mode));
     double part[kMaxRed];
     init_part(ph, part);
     PhaseWork<T, BLOCK, TNNZ> W(L, ss, reinterpret_cast<T*>(lds_raw), 0, 1);
@@ -159,50 +167,125 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
   }
 }
 
-// W workgroups share one LP's phase; each writes its partial sums.
-template <class T, int BLOCK, int TNNZ>
-__global__ void __launch_bounds__(BLOCK) pdlp_phase_kernel(const LpDev<T>* __restrict__ lp) {
-  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
-  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
-  const LpDev<T>& L = *lp;
-  const LpState s = *L.st;
-  const int ph = uniform(s.phase);
-  if (ph == PH_DONE) return;
-  double part[kMaxRed];
-  init_part(ph, part);
-  PhaseWork<T, BLOCK, TNNZ> W(L, s, reinterpret_cast<T*>(lds_raw), (int)blockIdx.x,
-                              (int)gridDim.x);
-  W.run(ph, part);
-  reduce_phase_to_lds<BLOCK>(ph, part, red);
-  constexpr int NW = BLOCK / kWave;
-  if (threadIdx.x < kMaxRed)
-    L.partials[(size_t)blockIdx.x * kMaxRed + threadIdx.x] = red[NW * kMaxRed + threadIdx.x];
+// Sum of the W partials of a phase in a fixed order (lane t: partials t,
+// t + BLOCK, ...; then the workgroup tree), the K values a phase's transition
+// reads: the totals land in red[(BLOCK/64)*kMaxRed + k].
+template <int BLOCK, int K>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ pin, int W, uint32_t mm,
+                                             double* red) {
+  double part[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) part[k] = ((mm >> k) & 1u) ? -__builtin_inf() : 0.0;
+  for (int b = threadIdx.x; b < W; b += BLOCK) {
+    const double* pb = pin + (size_t)b * kMaxRed;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      part[k] = ((mm >> k) & 1u) ? nanmax(part[k], pb[k]) : part[k] + pb[k];
+  }
+  block_reduce_to_lds<BLOCK, K>(part, mm, red);
 }
 
-// One workgroup: reduce the W partials of the phase just run (fixed order) and
-// take the transition.
+// W workgroups of one LP: the step kernels replace the phase + decide
+// launch pair by one launch.  Every workgroup first takes the pending
+// transition of the phase before it itself (the same partials summed in the
+// same order, so every workgroup computes the same state; only workgroup 0
+// writes history rows, counts the phase and stores the state), then runs the
+// new phase and writes its partials.  State and partials alternate between
+// two copies by launch parity: launch k reads copy k % 2 of the state and
+// partial half (k + 1) % 2 and writes the other ones, so no launch reads
+// what another workgroup of the same launch writes.
+template <class T, int BLOCK, int TNNZ>
+__global__ void __launch_bounds__(BLOCK)
+    pdlp_step_kernel(const LpDev<T>* __restrict__ lp, int parity, const StepCtl* __restrict__ ctl,
+                     unsigned* done) {
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
+  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
+  __shared__ LpState ss;
+  __shared__ int go;
+  const LpDev<T>& L = *lp;
+  const int W = (int)gridDim.x;
+  LpState* in = parity ? L.st2 : L.st;
+  LpState* out = parity ? L.st : L.st2;
+  const double* pin = L.partials + (size_t)(parity ^ 1) * W * kMaxRed;
+  double* pout = L.partials + (size_t)parity * W * kMaxRed;
+  const bool lead = blockIdx.x == 0;
+  if (threadIdx.x == 0) ss = *in;
+  __syncthreads();
+  if (uniform(ss.pending)) {
+    const int ph = uniform(ss.phase);
+    const uint32_t mm = phase_maxmask(ph);
+    if (ph == PH_CHECK) sum_partials<BLOCK, kMaxRed>(pin, W, mm, red);
+    else if (phase_reduces(ph, uniform(ss.next_mode))) sum_partials<BLOCK, 6>(pin, W, mm, red);
+    if (threadIdx.x == 0) {
+      const double* t = red + (BLOCK / kWave) * kMaxRed;
+      if (ph == PH_INIT) {
+        ss.tick0 = (long long)wall_clock64();
+        ss.ticks_left = ctl->ticks;
+      }
+      if (lead) {
+        L.pcount[ph] += 1;
+        if ((ph == PH_B_ADAPT || ph == PH_B_FUSED) && ss.next_mode != NEXT_NONE)
+          L.pcount[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
+      }
+      Transition tr{ctl->P, L.offset0, L.flip, lead};
+      ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap, L.rhist);
+      ss.pending = 0;
+      if (ss.phase == PH_DONE && lead) {
+        ss.ticks = (long long)wall_clock64() - ss.tick0;
+        atomicAdd(done, 1u);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) go = ss.phase != PH_DONE;
+  __syncthreads();
+  if (!uniform(go)) {
+    if (lead && threadIdx.x == 0) *out = ss;
+    return;
+  }
+  const int ph = uniform(ss.phase);
+  const bool reduce = phase_reduces(ph, uniform(ss.next_mode));
+  double part[kMaxRed];
+  init_part(ph, part);
+  PhaseWork<T, BLOCK, TNNZ> Wk(L, ss, reinterpret_cast<T*>(lds_raw), (int)blockIdx.x, W);
+  Wk.run(ph, part);
+  if (reduce) {
+    reduce_phase_to_lds<BLOCK>(ph, part, red);
+    constexpr int NW = BLOCK / kWave;
+    if (threadIdx.x < kMaxRed)
+      pout[(size_t)blockIdx.x * kMaxRed + threadIdx.x] = red[NW * kMaxRed + threadIdx.x];
+  } else {
+    __syncthreads();      // every thread is done with `ss` before workgroup 0 moves it
+  }
+  if (lead && threadIdx.x == 0) {
+    ss.pending = 1;
+    *out = ss;
+  }
+}
+
+// After a run of step kernels: takes the last phase's pending transition
+// (one workgroup, partial half `half`), so the state in L.st is current
+// between replays.
 template <class T, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-    pdlp_decide_kernel(const LpDev<T>* __restrict__ lp, int W, const StepCtl* __restrict__ ctl,
-                       unsigned* done) {
+    pdlp_settle_kernel(const LpDev<T>* __restrict__ lp, int W, int half,
+                       const StepCtl* __restrict__ ctl, unsigned* done) {
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   __shared__ LpState ss;
   const LpDev<T>& L = *lp;
-  const LpState s = *L.st;
-  const int ph = uniform(s.phase);
-  if (ph == PH_DONE) return;
+  if (threadIdx.x == 0) ss = *L.st;
+  __syncthreads();
+  if (!uniform(ss.pending)) return;
+  const int ph = uniform(ss.phase);
   const uint32_t mm = phase_maxmask(ph);
-  double part[kMaxRed];
-#pragma unroll
-  for (int k = 0; k < kMaxRed; ++k) part[k] = ((mm >> k) & 1u) ? -__builtin_inf() : 0.0;
-  for (int b = threadIdx.x; b < W; b += BLOCK) {
-    const double* pb = L.partials + (size_t)b * kMaxRed;
-#pragma unroll
-    for (int k = 0; k < kMaxRed; ++k)
-      part[k] = ((mm >> k) & 1u) ? nanmax(part[k], pb[k]) : part[k] + pb[k];
+  const double* pin = L.partials + (size_t)half * W * kMaxRed;
+  if (ph == PH_CHECK) sum_partials<BLOCK, kMaxRed>(pin, W, mm, red);
+  else if (phase_reduces(ph, uniform(ss.next_mode))) sum_partials<BLOCK, 6>(pin, W, mm, red);
+  if (threadIdx.x == 0) {
+    LpState s = ss;
+    s.pending = 0;
+    take_transition(L, s, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
   }
-  block_reduce_to_lds<BLOCK, kMaxRed>(part, mm, red);
-  if (threadIdx.x == 0) take_transition(L, s, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
 }
 
 // Puts every LP of a batch back at the start of its solve (a kernel rather

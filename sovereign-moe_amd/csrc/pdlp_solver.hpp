@@ -118,7 +118,8 @@ struct LpState {             // everything the loop carries between phases
   int nobetter;              // consecutive evaluations that did not improve on it
   int save_best;             // the next REF_SHIFT copies X, Y to xbest, ybest
   int use_best;              // report xbest, ybest rather than X, Y
-  int pad_best;
+  int pending;               // W workgroups: `phase` has run and its transition
+                             // (from the partial sums) is still to be taken
 };
 
 template <class T>
@@ -158,9 +159,11 @@ struct LpDev {
   double *xbest, *ybest;     // the refinement's best X, Y (FP32 engines)
   double* rhist;             // [kMaxRounds][5] per round: k, pres, dres, gap, pobj
   LpState* st;
+  LpState* st2;              // W workgroups: the other state of the step kernels' ping-pong
   long long* pcount;         // [kPhaseSlots] phase counters
   long long* pticks;         // [kPhaseSlots] device clock ticks spent in each phase (run kernel)
-  double* partials;          // [W][kMaxRed] per-workgroup partial sums (W-way phases)
+  double* partials;          // [2][W][kMaxRed] per-workgroup partial sums (W-way phases;
+                             // the step kernels alternate between the halves)
   // warm start (served resolves; null / 0 = the reference's cold start
   // x0 = clip(0), y0 = 0, omega0 from the norms): x and y in the units and
   // signs of the report (xo, yo), and a primal weight to start from
@@ -853,6 +856,7 @@ struct Transition {
   const Params& P;
   double offset0;            // minimisation-form offset
   int flip;                  // the caller's problem maximises
+  bool commit = true;        // write the history rows (false: only the state moves)
 
   __device__ __forceinline__ double merit(const LpState& s, double p, double d, double cx,
                                           double supp, double bnd, double w) const {
@@ -1087,7 +1091,7 @@ struct Transition {
           s.fin[0] = pu; s.fin[1] = du; s.fin[2] = gu; s.fin[3] = pou; s.fin[4] = dou;
           return use_avg ? PH_COPY_AVG : PH_FINAL;
         }
-        if (s.hist_len < hist_cap) {
+        if (commit && s.hist_len < hist_cap) {
           double* h = hist + 5 * s.hist_len;
           h[0] = (double)(s.k_done + s.k); h[1] = pu; h[2] = du; h[3] = gu; h[4] = pou;
         }
@@ -1125,8 +1129,10 @@ struct Transition {
         const double pou = t[2] + offset0, dou = (-t[3] + t[4]) + offset0;
         const double gap = fabs(pou - dou);
         if (s.ref_rows < kMaxRounds) {
-          double* h = rhist + 5 * s.ref_rows;
-          h[0] = (double)(s.k_done + s.k); h[1] = pres; h[2] = dres; h[3] = gap; h[4] = pou;
+          if (commit) {
+            double* h = rhist + 5 * s.ref_rows;
+            h[0] = (double)(s.k_done + s.k); h[1] = pres; h[2] = dres; h[3] = gap; h[4] = pou;
+          }
           ++s.ref_rows;
         }
         if (s.round == 0) s.k_round0 = s.k;
