@@ -273,11 +273,10 @@ int single_create_T(const sovlp_lp* lp, const sovlp_params* prm, void* ws, int64
     launch_run<T>(kBlock, d_lp, 1, ctl, done, kPhasesPerLaunch, h->cap);
     h->phases_per_replay = kPhasesPerLaunch;
   } else {
-    static_assert(kPairsPerGraph % 2 == 0, "the last step launch must write L.st");
     for (int i = 0; i < kPairsPerGraph; ++i)
       pdlp_step_kernel<T, kBlock, TN><<<W, kBlock, 0, h->cap>>>(d_lp, i & 1, ctl, done);
-    pdlp_settle_kernel<T, kBlock><<<1, kBlock, 0, h->cap>>>(d_lp, W, (kPairsPerGraph - 1) & 1,
-                                                            ctl, done);
+    pdlp_settle_kernel<T, kBlock><<<1, kBlock, 0, h->cap>>>(d_lp, W, kPairsPerGraph & 1,
+                                                            (kPairsPerGraph - 1) & 1, ctl, done);
     h->phases_per_replay = kPairsPerGraph;
   }
   HIPCHK(hipStreamEndCapture(h->cap, &h->graph));
@@ -478,7 +477,8 @@ size_t server_ws_T(int slots, int team) {
   cv.take<StepCtl>(1);
   cv.take<unsigned>(64);
   if (team > 1) {
-    cv.take<double>((size_t)slots * team * kMaxRed);
+    cv.take<LpState>(slots);
+    cv.take<double>((size_t)slots * 2 * team * kMaxRed);
     cv.take<int32_t>((size_t)slots * 2 * (team + 1));
   }
   return cv.off + 256;
@@ -504,8 +504,10 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
   StepCtl* ctl = cv.take<StepCtl>(1);
   unsigned* done = cv.take<unsigned>(64);
   h->team = team;
+  LpState* states2 = nullptr;
   if (team > 1) {
-    h->partials = cv.take<double>((size_t)slots * team * kMaxRed);
+    states2 = cv.take<LpState>(slots);
+    h->partials = cv.take<double>((size_t)slots * 2 * team * kMaxRed);
     h->parts = cv.take<int32_t>((size_t)slots * 2 * (team + 1));
     HIPCHK(hipHostMalloc((void**)&h->hparts, sizeof(int32_t) * (size_t)slots * 2 * (team + 1),
                          hipHostMallocDefault));
@@ -533,6 +535,7 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
   for (int i = 0; i < slots; ++i) {
     stage[i] = LpDev<T>{};
     stage[i].st = states + i;
+    stage[i].st2 = states2 ? states2 + i : nullptr;
     stage[i].pcount = pcount + (size_t)i * kPhaseSlots;
     stage[i].pticks = pticks + (size_t)i * kPhaseSlots;
     h->hinit[i] = LpState{};
@@ -557,13 +560,15 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
   HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
   if (team > 1) {
-    // `phases` pairs: one phase of every slot over `team` workgroups, then one
-    // workgroup per slot reduces the partial sums and takes the transition
+    // `phases` step launches: one phase of every slot over `team` workgroups,
+    // each workgroup first taking the slot's previous transition; then one
+    // workgroup per slot takes the last one
     constexpr int TN = sizeof(T) == 8 ? kTnnz64 : kTnnz32;
-    for (int i = 0; i < phases; ++i) {
-      pdlp_slot_phase_kernel<T, kBlock, TN><<<slots * team, kBlock, 0, h->cap>>>(d_lps, team);
-      pdlp_slot_decide_kernel<T, kBlock><<<slots, kBlock, 0, h->cap>>>(d_lps, team, ctl, done);
-    }
+    for (int i = 0; i < phases; ++i)
+      pdlp_slot_step_kernel<T, kBlock, TN><<<slots * team, kBlock, 0, h->cap>>>(d_lps, team, i & 1,
+                                                                              ctl, done);
+    pdlp_settle_kernel<T, kBlock><<<slots, kBlock, 0, h->cap>>>(d_lps, team, phases & 1,
+                                                                (phases - 1) & 1, ctl, done);
   } else {
     launch_run<T>(block, d_lps, slots, ctl, done, phases, h->cap);
   }
@@ -604,7 +609,8 @@ int server_admit_T(sovlp_server* h, int slot, const sovlp_lp* lp, const double* 
     HIPCHK(hipMemcpyAsync(dp, hp, sizeof(int32_t) * 2 * (W + 1), hipMemcpyHostToDevice, st));
     L.partA = dp;
     L.partT = dp + W + 1;
-    L.partials = h->partials + (size_t)slot * W * kMaxRed;
+    L.partials = h->partials + (size_t)slot * 2 * W * kMaxRed;
+    L.st2 = reinterpret_cast<LpDev<T>*>(h->hstage)[slot].st2;
   }
   L.wx0 = x0;
   L.wy0 = y0;

@@ -185,30 +185,25 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ pin, int
   block_reduce_to_lds<BLOCK, K>(part, mm, red);
 }
 
-// W workgroups of one LP: the step kernels replace the phase + decide
+// W workgroups of one LP: the step launches replace the phase + decide
 // launch pair by one launch.  Every workgroup first takes the pending
 // transition of the phase before it itself (the same partials summed in the
-// same order, so every workgroup computes the same state; only workgroup 0
-// writes history rows, counts the phase and stores the state), then runs the
-// new phase and writes its partials.  State and partials alternate between
-// two copies by launch parity: launch k reads copy k % 2 of the state and
-// partial half (k + 1) % 2 and writes the other ones, so no launch reads
-// what another workgroup of the same launch writes.
+// same order, so every workgroup computes the same state; only the team's
+// lead, rank 0, writes history rows, counts the phase and stores the state),
+// then runs the new phase and writes its partials.  State and partials
+// alternate between two copies by launch parity: launch k reads state copy
+// k % 2 and partial half (k + 1) % 2 and writes the other ones, so no launch
+// reads what another workgroup of the same launch writes.  L.partials holds
+// the two halves, [2][W][kMaxRed].
 template <class T, int BLOCK, int TNNZ>
-__global__ void __launch_bounds__(BLOCK)
-    pdlp_step_kernel(const LpDev<T>* __restrict__ lp, int parity, const StepCtl* __restrict__ ctl,
-                     unsigned* done) {
-  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
-  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
-  __shared__ LpState ss;
-  __shared__ int go;
-  const LpDev<T>& L = *lp;
-  const int W = (int)gridDim.x;
+__device__ __forceinline__ void team_step(const LpDev<T>& L, int rank, int W, int parity,
+                                          const StepCtl* __restrict__ ctl, unsigned* done,
+                                          T* lds, double* red, LpState& ss, int& go) {
   LpState* in = parity ? L.st2 : L.st;
   LpState* out = parity ? L.st : L.st2;
   const double* pin = L.partials + (size_t)(parity ^ 1) * W * kMaxRed;
   double* pout = L.partials + (size_t)parity * W * kMaxRed;
-  const bool lead = blockIdx.x == 0;
+  const bool lead = rank == 0;
   if (threadIdx.x == 0) ss = *in;
   __syncthreads();
   if (uniform(ss.pending)) {
@@ -247,15 +242,15 @@ __global__ void __launch_bounds__(BLOCK)
   const bool reduce = phase_reduces(ph, uniform(ss.next_mode));
   double part[kMaxRed];
   init_part(ph, part);
-  PhaseWork<T, BLOCK, TNNZ> Wk(L, ss, reinterpret_cast<T*>(lds_raw), (int)blockIdx.x, W);
+  PhaseWork<T, BLOCK, TNNZ> Wk(L, ss, lds, rank, W);
   Wk.run(ph, part);
   if (reduce) {
     reduce_phase_to_lds<BLOCK>(ph, part, red);
     constexpr int NW = BLOCK / kWave;
     if (threadIdx.x < kMaxRed)
-      pout[(size_t)blockIdx.x * kMaxRed + threadIdx.x] = red[NW * kMaxRed + threadIdx.x];
+      pout[(size_t)rank * kMaxRed + threadIdx.x] = red[NW * kMaxRed + threadIdx.x];
   } else {
-    __syncthreads();      // every thread is done with `ss` before workgroup 0 moves it
+    __syncthreads();      // every thread is done with `ss` before the lead moves it
   }
   if (lead && threadIdx.x == 0) {
     ss.pending = 1;
@@ -263,19 +258,50 @@ __global__ void __launch_bounds__(BLOCK)
   }
 }
 
-// After a run of step kernels: takes the last phase's pending transition
-// (one workgroup, partial half `half`), so the state in L.st is current
-// between replays.
+// One LP over all gridDim.x workgroups.
+template <class T, int BLOCK, int TNNZ>
+__global__ void __launch_bounds__(BLOCK)
+    pdlp_step_kernel(const LpDev<T>* __restrict__ lp, int parity, const StepCtl* __restrict__ ctl,
+                     unsigned* done) {
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
+  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
+  __shared__ LpState ss;
+  __shared__ int go;
+  team_step<T, BLOCK, TNNZ>(*lp, (int)blockIdx.x, (int)gridDim.x, parity, ctl, done,
+                            reinterpret_cast<T*>(lds_raw), red, ss, go);
+}
+
+// Served LPs: slot blockIdx.x / team over `team` workgroups each (rank
+// blockIdx.x % team).
+template <class T, int BLOCK, int TNNZ>
+__global__ void __launch_bounds__(BLOCK)
+    pdlp_slot_step_kernel(const LpDev<T>* __restrict__ lps, int team, int parity,
+                          const StepCtl* __restrict__ ctl, unsigned* done) {
+  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
+  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
+  __shared__ LpState ss;
+  __shared__ int go;
+  const int slot = (int)blockIdx.x / team, rank = (int)blockIdx.x % team;
+  team_step<T, BLOCK, TNNZ>(lps[slot], rank, team, parity, ctl, done,
+                            reinterpret_cast<T*>(lds_raw), red, ss, go);
+}
+
+// After a run of step launches: one workgroup per LP (blockIdx.x) takes the
+// last phase's pending transition from state copy `from` (partial half
+// `half`) and leaves the state in L.st, current between replays.
 template <class T, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-    pdlp_settle_kernel(const LpDev<T>* __restrict__ lp, int W, int half,
+    pdlp_settle_kernel(const LpDev<T>* __restrict__ lps, int W, int from, int half,
                        const StepCtl* __restrict__ ctl, unsigned* done) {
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   __shared__ LpState ss;
-  const LpDev<T>& L = *lp;
-  if (threadIdx.x == 0) ss = *L.st;
+  const LpDev<T>& L = lps[blockIdx.x];
+  if (threadIdx.x == 0) ss = from ? *L.st2 : *L.st;
   __syncthreads();
-  if (!uniform(ss.pending)) return;
+  if (!uniform(ss.pending)) {
+    if (from && threadIdx.x == 0) *L.st = ss;
+    return;
+  }
   const int ph = uniform(ss.phase);
   const uint32_t mm = phase_maxmask(ph);
   const double* pin = L.partials + (size_t)half * W * kMaxRed;
@@ -303,59 +329,6 @@ __global__ void state_reset_kernel(const LpDev<T>* __restrict__ lps, int count, 
     lps[i].pcount[k] = 0;
     lps[i].pticks[k] = 0;
   }
-}
-
-
-}  // namespace sov
-
-namespace sov {
-
-// Served LPs over `team` workgroups each (slot = blockIdx.x / team, rank =
-// blockIdx.x % team): one phase of every occupied slot, partial sums per rank.
-template <class T, int BLOCK, int TNNZ>
-__global__ void __launch_bounds__(BLOCK)
-    pdlp_slot_phase_kernel(const LpDev<T>* __restrict__ lps, int team) {
-  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
-  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
-  const int slot = (int)blockIdx.x / team, rank = (int)blockIdx.x % team;
-  const LpDev<T>& L = lps[slot];
-  const LpState s = *L.st;
-  const int ph = uniform(s.phase);
-  if (ph == PH_DONE) return;
-  double part[kMaxRed];
-  init_part(ph, part);
-  PhaseWork<T, BLOCK, TNNZ> W(L, s, reinterpret_cast<T*>(lds_raw), rank, team);
-  W.run(ph, part);
-  reduce_phase_to_lds<BLOCK>(ph, part, red);
-  constexpr int NW = BLOCK / kWave;
-  if (threadIdx.x < kMaxRed)
-    L.partials[(size_t)rank * kMaxRed + threadIdx.x] = red[NW * kMaxRed + threadIdx.x];
-}
-
-// One workgroup per slot: reduce the team's partials (fixed order) and take
-// the transition.
-template <class T, int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
-    pdlp_slot_decide_kernel(const LpDev<T>* __restrict__ lps, int team,
-                            const StepCtl* __restrict__ ctl, unsigned* done) {
-  __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
-  __shared__ LpState ss;
-  const LpDev<T>& L = lps[blockIdx.x];
-  const LpState s = *L.st;
-  const int ph = uniform(s.phase);
-  if (ph == PH_DONE) return;
-  const uint32_t mm = phase_maxmask(ph);
-  double part[kMaxRed];
-#pragma unroll
-  for (int k = 0; k < kMaxRed; ++k) part[k] = ((mm >> k) & 1u) ? -__builtin_inf() : 0.0;
-  for (int b = threadIdx.x; b < team; b += BLOCK) {
-    const double* pb = L.partials + (size_t)b * kMaxRed;
-#pragma unroll
-    for (int k = 0; k < kMaxRed; ++k)
-      part[k] = ((mm >> k) & 1u) ? nanmax(part[k], pb[k]) : part[k] + pb[k];
-  }
-  block_reduce_to_lds<BLOCK, kMaxRed>(part, mm, red);
-  if (threadIdx.x == 0) take_transition(L, s, ph, red + (BLOCK / kWave) * kMaxRed, ctl, done, ss);
 }
 
 }  // namespace sov
