@@ -1059,10 +1059,11 @@ struct Transition {
         if (refining()) {
           // a round stalls when its KKT ratio has not halved for
           // refine_stall iterations (FP32 iterates can stall short of the
-          // round's tolerance), or after 8x that.  If the best X met _converged, it is
-          // reported and the round dropped; otherwise the solve refines from
-          // the round's current candidate (a round that then cuts the gap by
-          // less than 10x makes the next one zoom out)
+          // round's tolerance), or after 8x that.  The round's current
+          // candidate is then evaluated like a converged round's (REF_KKT):
+          // the refinement goes on while evaluations improve on the best X
+          // and stops, reporting the best, after two that do not (a round
+          // that cuts the gap by less than 10x makes the next one zoom out)
           const double ratio = fmax(fmax(pu / s.ptol, du / s.dtol), gu / gtol);
           if (ratio < 0.5 * s.kkt_best) {
             s.kkt_best = ratio;
@@ -1073,17 +1074,8 @@ struct Transition {
           const long long stall_c = P.refine_stall > 0 ? P.refine_stall : 1024;
           const long long stall = s.best_q <= 1.0 ? stall_c
                                 : (P.refine_stall_open > 0 ? P.refine_stall_open : 4096);
-          if (ratio > 1.0 && (s.k - s.k_best >= stall || s.k >= 8 * stall)) {
-            if (s.round > 0 && s.best_q <= 1.0) {
-              // the best X evaluated so far met _converged: report it
-              s.absorbed = 1;
-              s.use_best = s.nobetter > 0;
-              for (int k = 0; k < 5; ++k) s.fin[k] = s.best_fin[k];
-              s.status = ST_OPTIMAL;
-              return PH_FINAL;
-            }
+          if (ratio > 1.0 && (s.k - s.k_best >= stall || s.k >= 8 * stall))
             return use_avg ? PH_COPY_AVG : PH_REF_X;
-          }
         }
         if (pu <= s.ptol && du <= s.dtol && gu <= gtol) {
           if (refining()) return use_avg ? PH_COPY_AVG : PH_REF_X;   // round converged

@@ -4,7 +4,7 @@ No reference counterpart: the reference launches one fixed shape, a 32-lane
 warp per row in 128-thread blocks (``src/sovopt/core/backend.py:373-384``).
 
 A configuration is ``(block, tile_rows, tile_nnz)``: the workgroup size that
-runs the LP (64, 128 or 256 threads; one compiled kernel each), and the caps of
+runs the LP (64, 128, 256 or 512 threads; one compiled kernel each), and the caps of
 the host tile plan — rows per tile (``block`` to ``4 block``: up to four short
 rows per lane) and nonzeros per tile (up to the kernel's LDS tile). LPs are
 bucketed by their nonzero count; for every bucket the tuner solves a sample of
@@ -33,8 +33,8 @@ from .batch import batch_geometry
 __all__ = ["Autotuner", "CANDIDATES", "NNZ_EDGES", "bucket_of", "packaged_table"]
 
 # (block, rows per lane, fraction of the kernel's tile nonzeros)
-CANDIDATES = [(256, 4, 1.0), (256, 1, 1.0), (256, 4, 0.5), (128, 4, 1.0), (128, 1, 1.0),
-              (64, 4, 1.0)]
+CANDIDATES = [(512, 4, 1.0), (512, 4, 0.5), (256, 4, 1.0), (256, 1, 1.0), (256, 4, 0.5),
+              (128, 4, 1.0), (128, 1, 1.0), (64, 4, 1.0)]
 # nonzero-count bucket edges
 NNZ_EDGES = [0, 1024, 4096, 16384, 1 << 62]
 
@@ -80,8 +80,20 @@ class Autotuner:
         return config_of(self.precision, 256, 4, 1.0)
 
     def choose(self, p: Problem):
-        """The configuration of LP ``p`` (a function of its shape only)."""
-        return tuple(self.table.get(bucket_of(p), self.default()))
+        """The configuration of LP ``p`` (a function of its shape only).  A
+        bucket the tuning sample left empty takes the configuration of the
+        nearest tuned bucket below it (LPs larger than any the sample held
+        behave most like the largest it held), else above it."""
+        b = bucket_of(p)
+        if b in self.table:
+            return tuple(self.table[b])
+        below = [k for k in self.table if k < b]
+        above = [k for k in self.table if k > b]
+        if below:
+            return tuple(self.table[max(below)])
+        if above:
+            return tuple(self.table[min(above)])
+        return self.default()
 
     def tune(self, problems: list[Problem], params, fill: int = 1024, sample: int = 64,
              log=print) -> dict:

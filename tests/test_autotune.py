@@ -23,7 +23,7 @@ def test_candidates_fit_their_kernels(precision):
     for cand in CANDIDATES:
         b, r, t = config_of(precision, *cand)
         rmax, tmax = batch_geometry(precision, b)
-        assert b in (64, 128, 256) and 1 <= r <= rmax and 64 <= t <= tmax
+        assert b in (64, 128, 256, 512) and 1 <= r <= rmax and 64 <= t <= tmax
 
 
 def test_table_round_trip_and_choice_by_shape():
@@ -35,7 +35,28 @@ def test_table_round_trip_and_choice_by_shape():
     p = uneven_batch_member(1234).problem
     # the choice is a function of the LP alone
     assert u.choose(p) == u.choose(p.copy())
-    assert u.choose(p) == u.table.get(bucket_of(p), u.default())
+    b = bucket_of(p)
+    assert u.choose(p) == (u.table[b] if b in u.table else u.table[max(k for k in u.table if k < b)])
+
+
+def test_empty_buckets_take_the_nearest_tuned_configuration():
+    from sovlp.core.problem import Problem
+    from sovlp.core.sparse import SparseMatrix
+    t = Autotuner(32)
+    t.table = {1: config_of(32, 256, 4, 1.0), 2: config_of(32, 512, 4, 1.0)}
+
+    def lp_with(nnz):
+        rows = np.arange(nnz) % 50
+        cols = np.arange(nnz) // 50
+        n = int(cols.max()) + 1
+        A = SparseMatrix.from_triplets(rows, cols, np.ones(nnz), 50, n)
+        return Problem(A=A, c=np.ones(n), row_lb=np.zeros(50), row_ub=np.full(50, 1e30),
+                       col_lb=np.zeros(n), col_ub=np.ones(n))
+
+    assert t.choose(lp_with(20000)) == t.table[2]       # bucket 3: the largest tuned below
+    assert t.choose(lp_with(500)) == t.table[1]         # bucket 0: none below, nearest above
+    assert t.choose(lp_with(2000)) == t.table[1]
+    assert Autotuner(32).choose(lp_with(500)) == Autotuner(32).default()
 
 
 def test_table_with_an_oversized_tile_is_refused():
