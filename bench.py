@@ -105,6 +105,9 @@ def parse():
                     help="skip the SpMV section (A and A^T of the 200k x 1M LP)")
     ap.add_argument("--no-large-lp", action="store_true",
                     help="skip the 200k x 1M LP solves (FP64, FP32 + refinement)")
+    ap.add_argument("--no-lds-matrix", action="store_true",
+                    help="keep every LP's matrix in memory (by default the run kernel holds "
+                         "an LP's matrix in LDS for the PDHG loop when it fits)")
     ap.add_argument("--panels", action="store_true",
                     help="SpMV section: also A x as eight XCD column panels")
     ap.add_argument("--large-lp-given", action="store_true",
@@ -640,7 +643,8 @@ def main():
     log(f"generated {a.lps} LPs in {time.perf_counter() - t:.1f} s "
         f"(sum nnz {sum(p.A.nnz for p in problems)})")
     par = PDLPParams(eps_abs=a.tol, eps_rel=a.tol, feas_cap=a.tol, device="gpu",
-                     refine_stall=a.refine_stall, refine_stall_open=a.refine_stall_open)
+                     refine_stall=a.refine_stall, refine_stall_open=a.refine_stall_open,
+                     lds_matrix=not a.no_lds_matrix)
     tuner = None
     if a.tuning != "none":
         from sovlp.lp.autotune import Autotuner, packaged_table
@@ -717,6 +721,7 @@ def main():
     blocks = sorted(set(int(b) for b in eng.db.blocks))
     kernel = " + ".join(f"pdlp_run_kernel<{'double' if a.precision == 64 else 'float'}, {b}, "
                         f"{batch_geometry(a.precision, b)[1]}>" for b in blocks)
+    n_resident = int(eng.resident().sum())
     eng.close()
     if fb is not None:
         fb.close()
@@ -740,10 +745,12 @@ def main():
                    "lps": a.lps, "tolerance": a.tol, "precision": a.precision,
                    "block": a.block, "parallelism": "one workgroup per LP",
                    "kernel_configs": "autotuned" if tuner is not None else f"block {a.block}",
+                   "lds_matrix": not a.no_lds_matrix,
                    "refinement": refinement_config(par, a.precision)},
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "optimal": opt,
+        "lds_resident_lps": n_resident,
         "lps_per_s_all_statuses": a.lps * a.steps / wall,
         "max_rel_obj_err_vs_closed_form": float(max(rel)) if rel else None,   # optimal LPs
         "rel_obj_err_p99": float(np.percentile(rel, 99)) if rel else None,

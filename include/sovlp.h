@@ -259,7 +259,13 @@ int sovlp_spmv_panels_f32(int panels, int rows, const int32_t *ptrs, const int32
  *                <= refine_tol (1 + ||finite row bounds||_inf) and the duality
  *                gap of X, Y is <= refine_tol (1 + |pobj| + |dobj|), after
  *                _converged (pdlp.py:523-530) holds for X on the original.
- * reserved       0 */
+ * flags          0, or SOVLP_FLAG_NO_LDS_MATRIX: the one-workgroup engines
+ *                (batch, single LP on one workgroup, serving slots) then keep
+ *                no LP's matrix in LDS.  By default an LP whose scaled matrix
+ *                (both orientations, 16-bit indices), plans and step vectors
+ *                fit the workgroup's staging LDS is held there for the PDHG
+ *                loop; the results are the same bit for bit either way. */
+#define SOVLP_FLAG_NO_LDS_MATRIX 1
 typedef struct {
   double eps_abs, eps_rel, feas_cap;
   int64_t max_iter;
@@ -268,7 +274,7 @@ typedef struct {
   double beta_sufficient, beta_necessary, beta_artificial, primal_weight_smoothing;
   int32_t refine_rounds, refine_stall;
   double refine_eps, refine_tol;
-  int32_t refine_stall_open, reserved;
+  int32_t refine_stall_open, flags;
 } sovlp_params;
 
 /* One LP in minimisation form (costs and offset already negated for a
@@ -394,6 +400,14 @@ typedef struct sovlp_batch sovlp_batch;
 /* Tile geometry of the batched engine for `precision` and `block` (64, 128,
  * 256 or 512 threads per LP): the row cap and nonzero cap of a tile. */
 int sovlp_batch_geometry(int precision, int block, int *tile_rows, int *tile_nnz);
+
+/* Whether the one-workgroup engines on `block` threads hold an LP of this
+ * shape (tiles_a / tiles_t: its tile counts for that block's geometry) in
+ * LDS for the PDHG loop (see sovlp_params.flags): *resident = 1 or 0.  No
+ * reference counterpart (the reference streams the matrix from memory in
+ * every product, backend.py:118-152). */
+int sovlp_lds_resident(int precision, int block, int m, int n, int64_t nnz, int tiles_a,
+                       int tiles_t, int *resident);
 
 /* Workspace for `count` LPs (host array of descriptors with device pointers). */
 int sovlp_batch_workspace_bytes(int count, const sovlp_lp *lps, int precision,

@@ -462,6 +462,7 @@ inline int check_params(const sovlp_params* p, int precision) {
     return fail(1, "refine_eps must be > 0 and refine_tol >= 0");
   if (p->refine_stall < 0 || p->refine_stall_open < 0)
     return fail(1, "refine_stall and refine_stall_open must be >= 0");
+  if (p->flags & ~kKnownFlags) return fail(1, "unknown bits in flags");
   return 0;
 }
 

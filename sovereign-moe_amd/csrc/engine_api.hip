@@ -215,6 +215,19 @@ int sovlp_batch_geometry(int precision, int block, int* tile_rows, int* tile_nnz
   return 0;
 }
 
+int sovlp_lds_resident(int precision, int block, int m, int n, int64_t nnz, int tiles_a,
+                       int tiles_t, int* resident) {
+  if (precision != 64 && precision != 32) return fail(1, "precision must be 64 or 32");
+  if (!batch_block_ok(block)) return fail(1, "block must be 64, 128, 256 or 512");
+  if (!resident || m < 0 || n < 0 || nnz < 0 || tiles_a < 0 || tiles_t < 0)
+    return fail(1, "bad arguments");
+  const int tn = batch_tnnz(precision, block);
+  *resident = (precision == 64 ? run_kernel_resident<double>(m, n, nnz, tiles_a, tiles_t, tn)
+                               : run_kernel_resident<float>(m, n, nnz, tiles_a, tiles_t, tn))
+                  ? 1 : 0;
+  return 0;
+}
+
 int sovlp_batch_workspace_bytes(int count, const sovlp_lp* lps, int precision,
                                 int64_t history_cap, int64_t* bytes) {
   if (count < 1 || !lps || !bytes) return fail(1, "bad batch arguments");

@@ -17,9 +17,20 @@ namespace sov {
 
 // LDS of a phase: the products of one tile, two vectors of T (the CHECK
 // phase) or one vector of doubles (scaling phases, FP64 check passes)
+template <class T>
+constexpr int lds_doubles_of(int tnnz) {
+  return sizeof(T) == 8 ? 2 * tnnz : tnnz;
+}
 template <class T, int TNNZ>
 constexpr int lds_doubles() {
-  return sizeof(T) == 8 ? 2 * TNNZ : TNNZ;
+  return lds_doubles_of<T>(TNNZ);
+}
+
+// Whether the run kernel of BLOCK threads (tile cap tnnz) holds an LP of this
+// shape in LDS: the rule it applies to each LP, for the host.
+template <class T>
+inline bool run_kernel_resident(int m, int n, long long nnz, int ntA, int ntT, int tnnz) {
+  return resident_fits<T>(m, n, nnz, ntA, ntT, 8LL * lds_doubles_of<T>(tnnz), tnnz);
 }
 
 constexpr long long kNoStop = 0x7fffffffffffffffLL;
@@ -104,11 +115,22 @@ __device__ __forceinline__ void transition_in_lds(const LpDev<T>& L, int ph, con
   }
 }
 
+// The phases of the PDHG loop: they read the matrix only through tile passes
+// over the scaled copies and write dx / dy only in their epilogues, so they
+// can run on an LP's resident copy in LDS (ResidentLds).
+__device__ __forceinline__ bool loop_phase(int ph) {
+  return ph == PH_P_TRIAL || ph == PH_A_TRIAL || ph == PH_B_ADAPT || ph == PH_A_FUSED ||
+         ph == PH_B_FUSED;
+}
+
 // One workgroup per LP (blockIdx.x): up to `max_phases` phases of the LP in a
 // row.  Register use is capped for 4 waves per SIMD (a few spills, measured
 // faster at the batch's p50 than 3 waves without them).  Phase k+1 reads what phase k wrote to global memory; the workgroup
 // barrier between them orders those accesses (all of a workgroup's waves share
 // one CU).  The loop is bounded, and the state goes back to HBM at the end.
+// An LP whose resident copy fits the staging LDS runs the loop phases on it:
+// the copy is filled before the first loop phase of the launch and again after
+// any other phase (which may stage its tiles in the same LDS).
 template <class T, int BLOCK, int TNNZ>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8)))
     pdlp_run_kernel(const LpDev<T>* __restrict__ lps, const StepCtl* __restrict__ ctl,
@@ -132,6 +154,14 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
   }
   __syncthreads();
   if (uniform(ss.phase) == PH_DONE) return;
+  const int m = uniform(L.m), n = uniform(L.n), ntA = uniform(L.ntA), ntT = uniform(L.ntT);
+  const int nnz = uniform(L.ap[m]);
+  const bool resident =
+      uniform(!(sp.flags & kFlagNoLdsMatrix) &&
+              resident_fits<T>(m, n, nnz, ntA, ntT, (long long)sizeof(lds_raw), TNNZ));
+  ResidentLds<T> rl{};
+  if (resident) rl = carve_resident<T>(lds_raw, m, n, nnz, ntA);
+  bool res_ok = false;       // the resident copy holds the current matrix, dx and dy
   // thread 0: when the current phase started (the end of the previous one)
   long long tp = threadIdx.x == 0 ? (long long)wall_clock64() : 0;
   for (int it = 0; it < max_phases; ++it) {
@@ -142,9 +172,17 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
     // for a step-size trial that follows it.  Read here, while no thread
     // writes the state: thread 0 changes it right after the phase.
     const bool reduce = phase_reduces(ph, uniform(ss.next_mode));
+    const bool on_res = resident && loop_phase(ph);
+    if (on_res && !res_ok) {
+      load_resident<T, BLOCK>(L, nnz, rl);
+      __syncthreads();
+      res_ok = true;
+    }
+    if (!on_res) res_ok = false;           // this phase may stage tiles over the copy
     double part[kMaxRed];
     init_part(ph, part);
-    PhaseWork<T, BLOCK, TNNZ> W(L, ss, reinterpret_cast<T*>(lds_raw), 0, 1);
+    PhaseWork<T, BLOCK, TNNZ> W(L, ss, reinterpret_cast<T*>(lds_raw), 0, 1,
+                                on_res ? &rl : nullptr);
     W.run(ph, part);
     if (reduce) reduce_phase_to_lds<BLOCK>(ph, part, red);
     if (threadIdx.x == 0) {

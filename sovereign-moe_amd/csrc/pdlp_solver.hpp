@@ -58,8 +58,11 @@ struct Params {              // same layout as sovlp_params in include/sovlp.h
   double refine_tol;         // primal residual target: refine_tol (1 + ||b||_inf)
   int refine_stall_open;     // refine_stall while no evaluated X has met _converged
                              // (0: 4096)
-  int pad_;
+  int flags;                 // kFlagNoLdsMatrix: the run kernel keeps no matrix in LDS
 };
+
+constexpr int kFlagNoLdsMatrix = 1;
+constexpr int kKnownFlags = kFlagNoLdsMatrix;
 
 constexpr int kMaxRounds = 16;   // refinement rounds recorded per LP
 // Trust region of a refinement round, in its zoomed units: every finite
@@ -171,6 +174,77 @@ struct LpDev {
   double womega;
 };
 
+// ---------------------------------------------------------------------------
+// An LP's scaled matrix in both orientations (values, 16-bit column indices,
+// row pointers and tile plans) and its two step vectors dx, dy, held in the
+// run kernel's LDS when they fit.  The phases of the PDHG loop (P_TRIAL,
+// A_TRIAL, B_ADAPT, A_FUSED, B_FUSED) then read the matrix and gather the
+// step vectors from LDS; each of them that writes dx or dy writes both
+// copies.  Any other phase may use the same LDS for staging, so the copy is
+// reloaded from global memory before the next loop phase after it.  The
+// products and sums are those of the staged tile pass bit for bit.
+// ---------------------------------------------------------------------------
+template <class T>
+struct ResidentLds {
+  SOV_LDS T *valA, *valT, *dx, *dy;
+  SOV_LDS uint16_t *idxA, *idxT, *ptrA, *ptrT, *tilesA, *tilesT;
+};
+
+// LDS bytes of an LP's resident copy
+template <class T>
+__host__ __device__ inline long long resident_bytes(int m, int n, long long nnz, int ntA,
+                                                    int ntT) {
+  return (2 * nnz + m + n) * (long long)sizeof(T) +
+         2LL * (2 * nnz + (m + 1) + (n + 1) + (ntA + 1) + (ntT + 1));
+}
+
+// Whether an LP's copy fits `cap` bytes of LDS and 16-bit indices, with every
+// tile a short-row tile (at most tnnz nonzeros).
+template <class T>
+__host__ __device__ inline bool resident_fits(int m, int n, long long nnz, int ntA, int ntT,
+                                              long long cap, int tnnz) {
+  return m < 65536 && n < 65536 && nnz < 65536 && nnz <= tnnz &&
+         resident_bytes<T>(m, n, nnz, ntA, ntT) <= cap;
+}
+
+// `base`: the start of the LDS region (16-byte aligned)
+template <class T>
+__device__ inline ResidentLds<T> carve_resident(void* base, int m, int n, int nnz, int ntA) {
+  ResidentLds<T> r;
+  SOV_LDS T* t = (SOV_LDS T*)base;
+  r.valA = t;
+  r.valT = t + nnz;
+  r.dx = t + 2 * nnz;
+  r.dy = t + 2 * nnz + n;
+  SOV_LDS uint16_t* u = (SOV_LDS uint16_t*)(t + 2 * nnz + n + m);
+  r.idxA = u;
+  r.idxT = u + nnz;
+  r.ptrA = u + 2 * nnz;
+  r.ptrT = r.ptrA + (m + 1);
+  r.tilesA = r.ptrT + (n + 1);
+  r.tilesT = r.tilesA + (ntA + 1);
+  return r;
+}
+
+// Every thread of the workgroup: fill the resident copy from global memory
+// (the scaled matrix, its plans, and the current dx and dy).  The caller
+// places a workgroup barrier before the copy is read.
+template <class T, int BLOCK>
+__device__ inline void load_resident(const LpDev<T>& l, int nnz, const ResidentLds<T>& r) {
+  for (int q = (int)threadIdx.x; q < nnz; q += BLOCK) {
+    r.valA[q] = l.sav[q];
+    r.valT[q] = l.stv[q];
+    r.idxA[q] = (uint16_t)l.ai[q];
+    r.idxT[q] = (uint16_t)l.ti[q];
+  }
+  for (int i = (int)threadIdx.x; i <= l.m; i += BLOCK) r.ptrA[i] = (uint16_t)l.ap[i];
+  for (int j = (int)threadIdx.x; j <= l.n; j += BLOCK) r.ptrT[j] = (uint16_t)l.tp[j];
+  for (int t = (int)threadIdx.x; t <= l.ntA; t += BLOCK) r.tilesA[t] = (uint16_t)l.tilesA[t];
+  for (int t = (int)threadIdx.x; t <= l.ntT; t += BLOCK) r.tilesT[t] = (uint16_t)l.tilesT[t];
+  for (int j = (int)threadIdx.x; j < l.n; j += BLOCK) r.dx[j] = l.dx[j];
+  for (int i = (int)threadIdx.x; i < l.m; i += BLOCK) r.dy[i] = l.dy[i];
+}
+
 __device__ __forceinline__ double pow2_round(double s) {
   if (!(s > 0.0 && s <= 1.79769313486231570e308)) return 1.0;   // non-finite or <= 0
   return exp2(rint(log2(s)));
@@ -218,10 +292,12 @@ struct PhaseWork {
   T* lds;
   double* ldsd;
   int rank, size;
+  const ResidentLds<T>* res;   // the LP's resident copy in LDS (loop phases), or null
 
-  __device__ PhaseWork(const LpDev<T>& l_, const LpState& s_, T* lds_, int rank_, int size_)
+  __device__ PhaseWork(const LpDev<T>& l_, const LpState& s_, T* lds_, int rank_, int size_,
+                       const ResidentLds<T>* res_ = nullptr)
       : l(l_), s(s_), lds(lds_), ldsd(reinterpret_cast<double*>(lds_)), rank(rank_),
-        size(size_) {}
+        size(size_), res(res_) {}
 
   __device__ __forceinline__ void erange(long long len, long long* a, long long* b) const {
     *a = len * rank / size;
@@ -273,6 +349,7 @@ struct PhaseWork {
     const T v = clampT<T>(xj - tt * (l.c[j] + aty), l.lo[j], l.hi[j]);
     const T d = v - xj;
     l.dx[j] = d;
+    if (res) res->dx[j] = d;
     if (nm == NEXT_FUSED) {
       l.x[j] = v;
     } else {
@@ -479,6 +556,7 @@ struct PhaseWork {
           l.xn[j] = v;
           const T d = v - xj;
           l.dx[j] = d;
+          if (res) res->dx[j] = d;
           part[0] += (double)d * (double)d;
         }
       } break;
@@ -496,10 +574,15 @@ struct PhaseWork {
           l.yn[i] = yv;
           const T d = yv - r.y;
           l.dy[i] = d;
+          if (res) res->dy[i] = d;
           part[0] += (double)d * (double)a;
           part[1] += (double)d * (double)d;
         };
-        tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
+        if (res)
+          tile_pass_res<BLOCK, T>(res->valA, res->idxA, res->ptrA, res->tilesA, res->dx, a0, a1,
+                                  pre, epi);
+        else
+          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
       } break;
 
       case PH_B_ADAPT: {    // accepted trial
@@ -517,7 +600,11 @@ struct PhaseWork {
           l.sumx[j] = r.sumx + et * xj;
           next_primal(j, xj, aty, nm, tt, dxdx);
         };
-        tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
+        if (res)
+          tile_pass_res<BLOCK, T>(res->valT, res->idxT, res->ptrT, res->tilesT, res->dy, b0, b1,
+                                  pre, epi);
+        else
+          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
         for (long long i = i0 + tid; i < i1; i += BLOCK) {
           const T yv = l.yn[i];
           l.y[i] = yv;
@@ -538,11 +625,17 @@ struct PhaseWork {
           l.ax[i] = axn;
           const T w = r.y + sg * (axn + a);
           const T yv = dual_prox(w, r.q, sg, r.rl, r.ru);
-          l.dy[i] = yv - r.y;
+          const T d = yv - r.y;
+          l.dy[i] = d;
+          if (res) res->dy[i] = d;
           l.y[i] = yv;
           l.sumy[i] = r.sumy + et * yv;
         };
-        tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
+        if (res)
+          tile_pass_res<BLOCK, T>(res->valA, res->idxA, res->ptrA, res->tilesA, res->dx, a0, a1,
+                                  pre, epi);
+        else
+          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
       } break;
 
       case PH_B_FUSED: {    // aty += A^T dy; sum_x += eta x (backend.py:282-291); next primal
@@ -569,6 +662,7 @@ struct PhaseWork {
           const T v = clampT<T>(xj - tt * (r.c + aty), r.lo, r.hi);
           const T d = v - xj;
           l.dx[j] = d;
+          if (res) res->dx[j] = d;
           if (nm == NEXT_FUSED) {
             l.x[j] = v;
           } else {
@@ -576,7 +670,11 @@ struct PhaseWork {
             dxdx += (double)d * (double)d;
           }
         };
-        tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
+        if (res)
+          tile_pass_res<BLOCK, T>(res->valT, res->idxT, res->ptrT, res->tilesT, res->dy, b0, b1,
+                                  pre, epi);
+        else
+          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
         part[0] += dxdx;
       } break;
 

@@ -16,6 +16,10 @@
 
 namespace sov {
 
+// A pointer into LDS (address space 3): loads and stores through it are LDS
+// instructions, and it takes one 32-bit register.
+#define SOV_LDS __attribute__((address_space(3)))
+
 constexpr double kInf = 1.0e30;   // infinite-bound sentinel (tolerances.py:16)
 constexpr int kWave = 64;
 constexpr int kMaxRed = 24;       // values per team reduction
@@ -341,6 +345,98 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
     p0 = p1;
     r1 = nr1;
     p1 = np1;
+  }
+}
+
+// ------------------------------------------------------------------------
+// The tile pass over a matrix held in LDS (the run kernel keeps an LP's
+// scaled matrix there when it fits): val / idx / ptr / tiles are LDS copies
+// (16-bit indices and pointers), g the gathered vector (LDS too).  Each row
+// sum is formed in exactly the order, and from exactly the rounded products,
+// of tile_pass_pf's short-row branches, so the results are bit for bit those
+// of the staged pass: a product is rounded to T before it is added
+// (contraction off), as it is when staged through LDS.  There is nothing to
+// stage, so a tile needs no workgroup barrier; the only global loads are
+// pre()'s, issued before the products.  Every tile must be a short-row tile
+// (at most TNNZ nonzeros), which holds for any matrix of at most TNNZ
+// nonzeros.
+// ------------------------------------------------------------------------
+// Sum of the products val[q] * g[idx[q]] for q = a, a + step, ... < b, in
+// that order, each product rounded before it is added; four products at a
+// time, so that their LDS reads overlap.
+template <class T>
+__device__ __forceinline__ T lds_row_sum(const SOV_LDS T* __restrict__ val,
+                                         const SOV_LDS uint16_t* __restrict__ idx,
+                                         const SOV_LDS T* __restrict__ g, int a, int b,
+                                         int step, T s) {
+#pragma clang fp contract(off)
+  int q = a;
+  for (; q + 3 * step < b; q += 4 * step) {
+    const int j0 = idx[q], j1 = idx[q + step], j2 = idx[q + 2 * step], j3 = idx[q + 3 * step];
+    const T v0 = val[q], v1 = val[q + step], v2 = val[q + 2 * step], v3 = val[q + 3 * step];
+    const T p0 = v0 * g[j0], p1 = v1 * g[j1], p2 = v2 * g[j2], p3 = v3 * g[j3];
+    s = s + p0;
+    s = s + p1;
+    s = s + p2;
+    s = s + p3;
+  }
+  for (; q < b; q += step) {
+    const T p = val[q] * g[idx[q]];
+    s = s + p;
+  }
+  return s;
+}
+
+template <int BLOCK, class T, class Pre, class Epi>
+__device__ __forceinline__ void tile_pass_res(const SOV_LDS T* __restrict__ val,
+                                              const SOV_LDS uint16_t* __restrict__ idx,
+                                              const SOV_LDS uint16_t* __restrict__ ptr,
+                                              const SOV_LDS uint16_t* __restrict__ tiles,
+                                              const SOV_LDS T* __restrict__ g, int t0, int t1,
+                                              Pre& pre, Epi& epi) {
+#pragma clang fp contract(off)
+  for (int t = uniform(t0); t < uniform(t1); ++t) {
+    const int r0 = uniform(tiles[t]), r1 = uniform(tiles[t + 1]);
+    const int rows = r1 - r0;
+    if (rows > BLOCK) {
+      // one lane per row, up to kRowsPerLane rows per lane (tile_pass_pf's
+      // many-short-rows branch): the rows' operands first, then the sums
+      int a[kRowsPerLane], b[kRowsPerLane];
+      decltype(pre(0)) pr[kRowsPerLane];
+#pragma unroll
+      for (int k = 0; k < kRowsPerLane; ++k) {
+        const int row = (int)threadIdx.x + k * BLOCK;
+        const int rc = r0 + (row < rows ? row : rows - 1);
+        a[k] = ptr[rc];
+        b[k] = ptr[rc + 1];
+        pr[k] = pre(rc);
+      }
+#pragma unroll
+      for (int k = 0; k < kRowsPerLane; ++k) {
+        const int row = (int)threadIdx.x + k * BLOCK;
+        if (row < rows) {
+          T s[1] = {lds_row_sum<T>(val, idx, g, a[k], b[k], 1, T(0))};
+          epi(r0 + row, s, pr[k]);
+        }
+      }
+      for (int row = (int)threadIdx.x + kRowsPerLane * BLOCK; row < rows; row += BLOCK) {
+        const auto pq = pre(r0 + row);
+        T s[1] = {lds_row_sum<T>(val, idx, g, ptr[r0 + row], ptr[r0 + row + 1], 1, T(0))};
+        epi(r0 + row, s, pq);
+      }
+    } else {
+      int tpr = BLOCK / (rows > 0 ? rows : 1);
+      tpr = tpr >= 64 ? 64 : (tpr >= 32 ? 32 : (tpr >= 16 ? 16 : (tpr >= 8 ? 8 : (tpr >= 4 ? 4 : (tpr >= 2 ? 2 : 1)))));
+      const int grp = (int)threadIdx.x / tpr, ln = (int)threadIdx.x & (tpr - 1);
+      const bool active = grp < rows;
+      const int rc = r0 + (active ? grp : rows - 1);
+      const int a = ptr[rc], b = ptr[rc + 1];
+      const auto pr = pre(rc);
+      T s[1] = {T(0)};
+      if (active) s[0] = lds_row_sum<T>(val, idx, g, a + ln, b, tpr, T(0));
+      for (int o = tpr >> 1; o > 0; o >>= 1) s[0] = s[0] + __shfl_xor(s[0], o);
+      if (active && ln == 0) epi(r0 + grp, s, pr);
+    }
   }
 }
 

@@ -33,7 +33,7 @@ class sovlp_params(C.Structure):
                 ("beta_sufficient", F64), ("beta_necessary", F64), ("beta_artificial", F64),
                 ("primal_weight_smoothing", F64), ("refine_rounds", I32), ("refine_stall", I32),
                 ("refine_eps", F64), ("refine_tol", F64), ("refine_stall_open", I32),
-                ("reserved", I32)]
+                ("flags", I32)]
 
 
 class sovlp_lp(C.Structure):
@@ -130,6 +130,8 @@ _SIGS = {
     "sovlp_phase_name": (C.c_char_p, [C.c_int]),
     "sovlp_pdlp_phase_counts": (C.c_int, [P, P, P]),
     "sovlp_batch_geometry": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sovlp_lds_resident": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, I64, C.c_int, C.c_int,
+                                     C.POINTER(C.c_int)]),
     "sovlp_batch_workspace_bytes": (C.c_int, [C.c_int, P, C.c_int, I64, C.POINTER(I64)]),
     "sovlp_batch_create": (C.c_int, [C.c_int, P, C.POINTER(sovlp_params), C.c_int, C.c_int, P,
                                      I64, I64, P, C.POINTER(P)]),

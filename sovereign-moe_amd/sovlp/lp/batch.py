@@ -29,7 +29,8 @@ from ..core.backend import get_backend
 from ..core.problem import ObjSense, Problem, Solution, Status
 from .pdlp import PDLPInfo, PDLPParams, plan_tiles, refinement_rows
 
-__all__ = ["BatchResult", "BatchEngine", "solve_batch", "DeviceBatch", "batch_geometry"]
+__all__ = ["BatchResult", "BatchEngine", "solve_batch", "DeviceBatch", "batch_geometry",
+           "lds_resident"]
 
 
 def phases_per_launch() -> int:
@@ -43,6 +44,16 @@ def batch_geometry(precision: int, block: int) -> tuple[int, int]:
     r, t = C.c_int(), C.c_int()
     _lib.call("sovlp_batch_geometry", int(precision), int(block), C.byref(r), C.byref(t))
     return r.value, t.value
+
+
+def lds_resident(precision: int, block: int, m: int, n: int, nnz: int, tiles_a: int,
+                 tiles_t: int) -> bool:
+    """Whether the run kernel on ``block`` threads holds an LP of this shape in
+    LDS for the PDHG loop (the device's own rule)."""
+    r = C.c_int()
+    _lib.call("sovlp_lds_resident", int(precision), int(block), int(m), int(n), int(nnz),
+              int(tiles_a), int(tiles_t), C.byref(r))
+    return bool(r.value)
 
 
 class DeviceBatch:
@@ -198,6 +209,14 @@ class BatchEngine:
         out = np.zeros((self.count, slots.value), dtype=np.int64)
         _lib.call("sovlp_batch_phase_ticks", self.h, out.ctypes.data, self.stream)
         return out
+
+    def resident(self) -> np.ndarray:
+        """Per LP: whether its matrix is held in LDS for the PDHG loop (always
+        False with ``PDLPParams(lds_matrix=False)``)."""
+        if not self.params.lds_matrix:
+            return np.zeros(self.count, dtype=bool)
+        return np.array([lds_resident(self.precision, int(b), m, n, nnz, ta, tt)
+                         for (m, n, nnz, ta, tt, _o, _f), b in zip(self.db.meta, self.db.blocks)])
 
     def sizes(self) -> list[tuple[int, int, int, int, int]]:
         """(m, n, nnz, tiles of A, tiles of A^T) of every LP."""

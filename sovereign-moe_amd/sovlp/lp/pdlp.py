@@ -86,6 +86,13 @@ class PDLPParams:
     # LARGE_NNZ or more nonzeros (x no longer fits one XCD's L2 and the
     # gathers dominate A x), "given" below.
     column_order: str = "auto"
+    # The one-workgroup engines (batch, single LP on one workgroup, serving
+    # slots) hold an LP's scaled matrix, tile plans and step vectors in the
+    # workgroup's LDS for the PDHG loop when they fit its staging buffer (about
+    # 5000 nonzeros on 512-thread workgroups in FP32, 1300 on 128-thread ones):
+    # the loop then reads the matrix from LDS instead of memory.  The results
+    # are bit for bit the same; False keeps every matrix in memory.
+    lds_matrix: bool = True
 
     def resolved_refine(self, precision: int | None = None,
                         nnz: int | None = None) -> tuple[int, float]:
@@ -112,7 +119,8 @@ class PDLPParams:
                                  self.beta_necessary, self.beta_artificial,
                                  self.primal_weight_smoothing, rounds,
                                  int(self.refine_stall), eps,
-                                 float(self.refine_tol), int(self.refine_stall_open), 0)
+                                 float(self.refine_tol), int(self.refine_stall_open),
+                                 0 if self.lds_matrix else 1)
 
 
 @dataclass

@@ -120,3 +120,45 @@ def test_autotuned_mixed_configurations_in_one_batch():
         assert alone.iterations == mixed.solutions[i].iterations
         assert np.array_equal(alone.x, mixed.solutions[i].x)
         assert np.array_equal(alone.y, mixed.solutions[i].y)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_lds_resident_matrix_is_bit_for_bit_the_streamed_one(precision):
+    """The run kernel holds an LP's scaled matrix and step vectors in LDS when
+    they fit (the PDHG loop then reads no matrix from memory): every LP of a
+    mixed batch, resident or not, ends with the same iterations, x and y as with
+    every matrix streamed from memory (PDLPParams(lds_matrix=False)); the
+    single-LP engine on one workgroup does the same."""
+    from sovlp.lp.autotune import Autotuner, packaged_table
+    from sovlp.lp.batch import BatchEngine
+    print(f"[gpu] LDS-resident vs streamed matrix, FP{precision}", flush=True)
+    kos = _batch(48, base=7000)
+    probs = [k.problem for k in kos]
+    tuner = Autotuner.load(packaged_table(32)) if precision == 32 else None
+    out, resident = {}, None
+    for flag in (True, False):
+        par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6, device="gpu",
+                         lds_matrix=flag)
+        eng = BatchEngine(probs, par, precision=precision, block=256, tuner=tuner)
+        try:
+            eng.run()
+            out[flag], _ = eng.solutions()
+            if flag:
+                resident = eng.resident()
+        finally:
+            eng.close()
+    print(f"[gpu] {int(resident.sum())} of {len(probs)} LPs resident", flush=True)
+    assert 0 < resident.sum() < len(probs)
+    for i, (a, b) in enumerate(zip(out[True], out[False])):
+        assert a.status == b.status and a.iterations == b.iterations, i
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y), i
+        assert np.array_equal(a.reduced_costs, b.reduced_costs), i
+    i = int(np.flatnonzero(resident)[0])
+    single = [solve_pdlp(probs[i], PDLPParams(device="gpu", team="block", eps_abs=1e-6,
+                                              eps_rel=1e-6, precision=precision,
+                                              lds_matrix=flag)) for flag in (True, False)]
+    assert single[0].iterations == single[1].iterations
+    assert np.array_equal(single[0].x, single[1].x) and np.array_equal(single[0].y, single[1].y)
+    if precision == 64:      # same workgroup size as the batch: the batch's arithmetic
+        assert single[0].iterations == out[False][i].iterations
+        assert np.array_equal(single[0].x, out[False][i].x)
