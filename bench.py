@@ -42,8 +42,10 @@ Beside the headline number the JSON line carries:
                 (optional, --bnr)
   certify       the certificate of a point of the 200k x 1M LP (optional,
                 --certify)
-  serve         a Poisson stream of multi-period blending LPs through the
-                serving loop (optional, --serve)
+  serve         a Poisson stream of multi-period blending LPs (10 req/s for
+                10 s, T 24-36 hours) through the serving loop: FP64, 128 slots
+                of 8 workgroups, admission by free HBM; p50/p99 latency,
+                LPs/s, outcomes (--no-serve to skip)
 
 `--only SECTION` runs one section by itself (spmv, large_lp, bnr, certify,
 serve, or copy: a plain 256 MiB device copy whose counter readings calibrate
@@ -113,8 +115,8 @@ def parse():
     ap.add_argument("--large-lp-given", action="store_true",
                     help="large LP: also solve in the given column order at the "
                          "small-LP refinement tolerance")
-    ap.add_argument("--serve", action="store_true",
-                    help="also run a Poisson stream of blending LPs through the serving loop")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving section (a Poisson stream of blending LPs)")
     ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--serve-duration", type=float, default=10.0)
     ap.add_argument("--serve-tmin", type=int, default=24)
@@ -301,23 +303,33 @@ def certify_rate(reps: int = 20, k=None) -> dict:
     return out
 
 
-def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int, tuned: bool,
-                     refine: dict | None, replays: int):
-    """HBM bytes per launch of ``kernel`` on this workload from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's reading;
-    see profiles/), or None when they were taken on another configuration or
-    with another number of graph replays per step."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_pdlp_run_kernel_fp32_4096_lps.json")
-    try:
-        d = json.load(open(path))
-    except (OSError, ValueError):
-        return None
+def traffic_key(lps: int, seed: int, precision: int, block: int, tuned: bool,
+                refine: dict | None, lds_matrix: bool = True) -> str:
+    """The workload a traffic record was measured on, as the record names it."""
     want = f"{lps} uneven LPs, seeds {seed}..{seed + lps - 1}, " \
            f"{'FP32 + refinement' if precision == 32 else 'FP64'}, block {block}" \
            f"{', autotuned' if tuned else ''}"
     if refine:
         want += f", {refine['rounds']} rounds at {refine['round_eps']:g}, stall " \
                 f"{refine['stall_open']}/{refine['stall']}"
+    if lds_matrix:
+        want += ", matrices in LDS where they fit"
+    return want
+
+
+def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int, tuned: bool,
+                     refine: dict | None, replays: int, lds_matrix: bool = True):
+    """HBM bytes per launch of ``kernel`` on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's reading;
+    see profiles/, assembled by ``python -m sovlp.lp.traffic``), or None when
+    they were taken on another configuration or with another number of graph
+    replays per step."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_pdlp_run_kernel_fp32_4096_lps.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    want = traffic_key(lps, seed, precision, block, tuned, refine, lds_matrix)
     if d.get("kernel") != kernel or d.get("workload") != want \
             or d.get("replays_per_step") != replays:
         return None
@@ -765,7 +777,11 @@ def main():
                      "traffic": measured_traffic(kernel, a.lps, a.seed, a.precision, a.block,
                                                  tuner is not None,
                                                  refinement_config(par, a.precision),
-                                                 int(graphs[-1])),
+                                                 int(graphs[-1]), not a.no_lds_matrix),
+                     "traffic_key": traffic_key(a.lps, a.seed, a.precision, a.block,
+                                                tuner is not None,
+                                                refinement_config(par, a.precision),
+                                                not a.no_lds_matrix),
                      "algorithmic_bytes_per_launch": step_bytes / graphs[-1],
                      "avg_launch_us": launch_ms * 1e3, "launches": launches},
     }
@@ -795,7 +811,7 @@ def main():
         out["spmv"] = spmv_section()
     if not a.no_large_lp:
         out["large_lp"] = large_lp_section(large(), a.large_lp_given)
-    if a.serve:
+    if not a.no_serve:
         out["serve"] = serve_section(a)
         log(f"serve: {out['serve']}")
     if a.bnr:

@@ -181,8 +181,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
     if (!on_res) res_ok = false;           // this phase may stage tiles over the copy
     double part[kMaxRed];
     init_part(ph, part);
-    PhaseWork<T, BLOCK, TNNZ> W(L, ss, reinterpret_cast<T*>(lds_raw), 0, 1,
-                                on_res ? &rl : nullptr);
+    PhaseWork<T, BLOCK, TNNZ> W(L, ss, reinterpret_cast<T*>(lds_raw), 0, 1, rl, on_res);
     W.run(ph, part);
     if (reduce) reduce_phase_to_lds<BLOCK>(ph, part, red);
     if (threadIdx.x == 0) {

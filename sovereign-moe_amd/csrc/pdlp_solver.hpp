@@ -292,12 +292,15 @@ struct PhaseWork {
   T* lds;
   double* ldsd;
   int rank, size;
-  const ResidentLds<T>* res;   // the LP's resident copy in LDS (loop phases), or null
+  // the LP's resident copy in LDS (loop phases) when has_res; held by value so
+  // that its pointers stay in registers
+  ResidentLds<T> res;
+  bool has_res;
 
   __device__ PhaseWork(const LpDev<T>& l_, const LpState& s_, T* lds_, int rank_, int size_,
-                       const ResidentLds<T>* res_ = nullptr)
+                       const ResidentLds<T>& res_ = ResidentLds<T>{}, bool has_res_ = false)
       : l(l_), s(s_), lds(lds_), ldsd(reinterpret_cast<double*>(lds_)), rank(rank_),
-        size(size_), res(res_) {}
+        size(size_), res(res_), has_res(has_res_) {}
 
   __device__ __forceinline__ void erange(long long len, long long* a, long long* b) const {
     *a = len * rank / size;
@@ -349,7 +352,7 @@ struct PhaseWork {
     const T v = clampT<T>(xj - tt * (l.c[j] + aty), l.lo[j], l.hi[j]);
     const T d = v - xj;
     l.dx[j] = d;
-    if (res) res->dx[j] = d;
+    if (has_res) res.dx[j] = d;
     if (nm == NEXT_FUSED) {
       l.x[j] = v;
     } else {
@@ -556,7 +559,7 @@ struct PhaseWork {
           l.xn[j] = v;
           const T d = v - xj;
           l.dx[j] = d;
-          if (res) res->dx[j] = d;
+          if (has_res) res.dx[j] = d;
           part[0] += (double)d * (double)d;
         }
       } break;
@@ -574,12 +577,12 @@ struct PhaseWork {
           l.yn[i] = yv;
           const T d = yv - r.y;
           l.dy[i] = d;
-          if (res) res->dy[i] = d;
+          if (has_res) res.dy[i] = d;
           part[0] += (double)d * (double)a;
           part[1] += (double)d * (double)d;
         };
-        if (res)
-          tile_pass_res<BLOCK, T>(res->valA, res->idxA, res->ptrA, res->tilesA, res->dx, a0, a1,
+        if (has_res)
+          tile_pass_res<BLOCK, T>(res.valA, res.idxA, res.ptrA, res.tilesA, res.dx, a0, a1,
                                   pre, epi);
         else
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
@@ -600,8 +603,8 @@ struct PhaseWork {
           l.sumx[j] = r.sumx + et * xj;
           next_primal(j, xj, aty, nm, tt, dxdx);
         };
-        if (res)
-          tile_pass_res<BLOCK, T>(res->valT, res->idxT, res->ptrT, res->tilesT, res->dy, b0, b1,
+        if (has_res)
+          tile_pass_res<BLOCK, T>(res.valT, res.idxT, res.ptrT, res.tilesT, res.dy, b0, b1,
                                   pre, epi);
         else
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
@@ -627,12 +630,12 @@ struct PhaseWork {
           const T yv = dual_prox(w, r.q, sg, r.rl, r.ru);
           const T d = yv - r.y;
           l.dy[i] = d;
-          if (res) res->dy[i] = d;
+          if (has_res) res.dy[i] = d;
           l.y[i] = yv;
           l.sumy[i] = r.sumy + et * yv;
         };
-        if (res)
-          tile_pass_res<BLOCK, T>(res->valA, res->idxA, res->ptrA, res->tilesA, res->dx, a0, a1,
+        if (has_res)
+          tile_pass_res<BLOCK, T>(res.valA, res.idxA, res.ptrA, res.tilesA, res.dx, a0, a1,
                                   pre, epi);
         else
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
@@ -662,7 +665,7 @@ struct PhaseWork {
           const T v = clampT<T>(xj - tt * (r.c + aty), r.lo, r.hi);
           const T d = v - xj;
           l.dx[j] = d;
-          if (res) res->dx[j] = d;
+          if (has_res) res.dx[j] = d;
           if (nm == NEXT_FUSED) {
             l.x[j] = v;
           } else {
@@ -670,8 +673,8 @@ struct PhaseWork {
             dxdx += (double)d * (double)d;
           }
         };
-        if (res)
-          tile_pass_res<BLOCK, T>(res->valT, res->idxT, res->ptrT, res->tilesT, res->dy, b0, b1,
+        if (has_res)
+          tile_pass_res<BLOCK, T>(res.valT, res.idxT, res.ptrT, res.tilesT, res.dy, b0, b1,
                                   pre, epi);
         else
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);

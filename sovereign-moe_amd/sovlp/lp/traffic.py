@@ -26,7 +26,7 @@ import numpy as np
 
 from .. import _lib
 
-__all__ = ["phase_names", "phase_bytes", "bytes_of_counts"]
+__all__ = ["phase_names", "phase_bytes", "bytes_of_counts", "traffic_record", "counter_per_kernel"]
 
 _F64 = 8
 _IDX = 4
@@ -110,3 +110,118 @@ def bytes_of_counts(counts: np.ndarray, sizes, precision: int) -> float:
         per_lp = np.array([phase_bytes(names[k], *sz, precision) for sz in sizes], np.float64)
         total += float(counts[:, k] @ per_lp)
     return total
+
+
+# ---------------------------------------------------------------------------
+# The traffic record bench.py reports as a kernel's `traffic`: rocprofv3
+# counter readings (FETCH_SIZE, WRITE_SIZE, in KiB per dispatch) of the
+# profiled kernels, each scaled by what the same counter reads for a plain
+# device copy of known size.
+#
+#   python -m sovlp.lp.traffic --bench-json B --fetch F --write W \
+#       --copy-fetch CF --copy-write CW --out profiles/<record>.json
+#
+# B is the JSON line of the profiled bench run, F / W the output directories of
+# its two counter passes (`rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`,
+# csv output), CF / CW the same two passes over `bench.py --only copy`.
+# ---------------------------------------------------------------------------
+
+def counter_rows(path: str) -> list[dict]:
+    """Every row of the *counter_collection.csv files under ``path``."""
+    import csv
+    import glob
+    import os
+    rows = []
+    files = [path] if os.path.isfile(path) else sorted(
+        glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True))
+    for f in files:
+        with open(f, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    if not rows:
+        raise FileNotFoundError(f"no counter_collection.csv rows under {path}")
+    return rows
+
+
+def counter_per_kernel(path: str, counter: str) -> dict[str, tuple[float, int]]:
+    """(sum of the counter over dispatches, dispatches) per kernel name."""
+    out: dict[str, list] = {}
+    for r in counter_rows(path):
+        if r.get("Counter_Name") != counter:
+            continue
+        k = r["Kernel_Name"]
+        acc = out.setdefault(k, [0.0, set()])
+        acc[0] += float(r["Counter_Value"])
+        acc[1].add(r.get("Dispatch_Id") or r.get("Correlation_Id"))
+    return {k: (v[0], len(v[1])) for k, v in out.items()}
+
+
+def copy_scale(path: str, counter: str, nbytes: int, match: str = "copy") -> tuple[float, float]:
+    """(KiB per copy dispatch, bytes moved / KiB read) of the copy kernels
+    (kernel names containing ``match``) of a ``bench.py --only copy`` run."""
+    per = {k: v for k, v in counter_per_kernel(path, counter).items() if match in k.lower()}
+    if not per:
+        raise ValueError(f"no kernel named like '{match}' with {counter} under {path}")
+    total = sum(v[0] for v in per.values())
+    disp = sum(v[1] for v in per.values())
+    kib = total / disp
+    return kib, nbytes / (kib * 1024.0)
+
+
+def traffic_record(bench_line: dict, fetch: str, write: str, copy_fetch: str, copy_write: str,
+                   copy_bytes: int = 256 << 20) -> dict:
+    """The record for the bench line's dominant kernels (the roofline's
+    `kernel`, several joined by ' + '): their counter sums over the profiled
+    step, scaled by the copy calibration, per graph replay."""
+    roof = bench_line["roofline"]
+    names = [k.strip() for k in roof["kernel"].split(" + ")]
+    replays = roof["launches"] // bench_line["steps"]
+
+    def pick(path, counter):
+        per = counter_per_kernel(path, counter)
+        got = {}
+        for n in names:
+            hits = {k: v for k, v in per.items() if n in k}
+            if not hits:
+                raise ValueError(f"kernel {n} has no {counter} rows under {path}")
+            got[n] = sum(v[0] for v in hits.values())
+        return got
+
+    f, w = pick(fetch, "FETCH_SIZE"), pick(write, "WRITE_SIZE")
+    fk, fs = copy_scale(copy_fetch, "FETCH_SIZE", copy_bytes)
+    wk, ws = copy_scale(copy_write, "WRITE_SIZE", copy_bytes)
+    per_launch = (sum(f.values()) * fs + sum(w.values()) * ws) * 1024.0 / (
+        replays * bench_line["steps"])
+    return {
+        "kernel": roof["kernel"],
+        "workload": roof["traffic_key"],
+        "replays_per_step": replays,
+        "fetch_size_kib_per_step": {k: v / bench_line["steps"] for k, v in f.items()},
+        "write_size_kib_per_step": {k: v / bench_line["steps"] for k, v in w.items()},
+        "copy_calibration": {"bytes_read_and_written": copy_bytes, "fetch_size_kib": fk,
+                             "write_size_kib": wk, "fetch_scale": fs, "write_scale": ws},
+        "traffic_bytes_per_launch": per_launch,
+        "algorithmic_bytes_per_launch": roof["algorithmic_bytes_per_launch"],
+    }
+
+
+def _main(argv=None) -> int:
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="assemble the traffic record of a bench run")
+    ap.add_argument("--bench-json", required=True, help="file whose last line is the bench JSON")
+    for k in ("fetch", "write", "copy-fetch", "copy-write"):
+        ap.add_argument(f"--{k}", required=True)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    line = json.loads(open(a.bench_json).read().strip().splitlines()[-1])
+    rec = traffic_record(line, a.fetch, a.write, a.copy_fetch, a.copy_write)
+    with open(a.out, "w") as fh:
+        json.dump(rec, fh, indent=1)
+        fh.write("\n")
+    print(json.dumps({k: rec[k] for k in ("traffic_bytes_per_launch",
+                                          "algorithmic_bytes_per_launch")}))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())

@@ -230,6 +230,8 @@ def test_bench_reports_traffic_only_for_the_measured_configuration():
                                   d["replays_per_step"]) is None
     assert bench.measured_traffic(*args[:6], bench.refinement_config(PDLPParams(), 32),
                                   d["replays_per_step"]) is None
+    assert bench.measured_traffic(*args, d["replays_per_step"], False) is None
+    assert d["workload"] == bench.traffic_key(4096, 1000, 32, 256, True, refine, True)
     assert 0.5 < d["traffic_bytes_per_launch"] / d["algorithmic_bytes_per_launch"] < 1.5
 
 
