@@ -524,6 +524,10 @@ int sovlp_server_outputs(sovlp_server *h, int slot, double **x, double **y, doub
 
 /* Device pointer of the slot's refinement record (as sovlp_pdlp_refinement). */
 int sovlp_server_refinement(sovlp_server *h, int slot, double **rows);
+/* The slot's check history (device pointer; as sovlp_pdlp_outputs' history:
+ * min(info.history_len, history_cap) rows of (iteration, unscaled primal
+ * residual, dual residual, gap, primal objective)), as pdlp.py:433-441 logs it. */
+int sovlp_server_history(sovlp_server *h, int slot, double **rows);
 
 /* counts[sovlp_phase_slots()] of the slot's current LP; synchronises. */
 int sovlp_server_phase_counts(sovlp_server *h, int slot, int64_t *counts,

@@ -77,7 +77,7 @@ struct sovlp_server {
   LpState* hinit;            // pinned [slots]: fresh states being copied in
   LpState* hsnap;            // pinned [slots]: states after the last step
   std::vector<char> busy, seen_done;
-  std::vector<double*> xo, yo, dout, rhist;
+  std::vector<double*> xo, yo, dout, rhist, hist;
   hipGraph_t graph;
   hipGraphExec_t exec;
   hipStream_t cap;
@@ -529,6 +529,7 @@ int server_create_T(int slots, int team, const sovlp_params* prm, int block, int
   h->yo.assign(slots, nullptr);
   h->dout.assign(slots, nullptr);
   h->rhist.assign(slots, nullptr);
+  h->hist.assign(slots, nullptr);
   HIPCHK(hipHostMalloc((void**)&h->hstage, sizeof(LpDev<T>) * (size_t)slots, hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void**)&h->hinit, sizeof(LpState) * (size_t)slots, hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void**)&h->hsnap, sizeof(LpState) * (size_t)slots, hipHostMallocDefault));
@@ -630,6 +631,7 @@ int server_admit_T(sovlp_server* h, int slot, const sovlp_lp* lp, const double* 
   h->yo[slot] = L.yo;
   h->dout[slot] = L.dout;
   h->rhist[slot] = L.rhist;
+  h->hist[slot] = L.hist;
   return 0;
 }
 

@@ -222,8 +222,9 @@ int sovlp_lds_resident(int precision, int block, int m, int n, int64_t nnz, int 
   if (!resident || m < 0 || n < 0 || nnz < 0 || tiles_a < 0 || tiles_t < 0)
     return fail(1, "bad arguments");
   const int tn = batch_tnnz(precision, block);
-  *resident = (precision == 64 ? run_kernel_resident<double>(m, n, nnz, tiles_a, tiles_t, tn)
-                               : run_kernel_resident<float>(m, n, nnz, tiles_a, tiles_t, tn))
+  *resident = (precision == 64
+                   ? run_kernel_resident<double>(m, n, nnz, tiles_a, tiles_t, block, tn)
+                   : run_kernel_resident<float>(m, n, nnz, tiles_a, tiles_t, block, tn))
                   ? 1 : 0;
   return 0;
 }
@@ -488,6 +489,13 @@ int sovlp_server_refinement(sovlp_server* h, int slot, double** rows) {
   return 0;
 }
 
+int sovlp_server_history(sovlp_server* h, int slot, double** rows) {
+  if (!h || !rows) return fail(1, "null argument");
+  if (slot < 0 || slot >= h->slots || !h->busy[slot]) return fail(1, "slot holds no LP");
+  *rows = h->hist[slot];
+  return 0;
+}
+
 int sovlp_server_phase_counts(sovlp_server* h, int slot, int64_t* counts,
                               sovlp_stream_t stream) {
   if (!h || !counts) return fail(1, "null argument");
@@ -504,7 +512,7 @@ int sovlp_server_release(sovlp_server* h, int slot) {
   if (!h->seen_done[slot]) return fail(1, "the slot's LP has not finished (step until it has)");
   h->busy[slot] = 0;
   h->seen_done[slot] = 0;
-  h->xo[slot] = h->yo[slot] = h->dout[slot] = h->rhist[slot] = nullptr;
+  h->xo[slot] = h->yo[slot] = h->dout[slot] = h->rhist[slot] = h->hist[slot] = nullptr;
   return 0;
 }
 

@@ -26,11 +26,25 @@ constexpr int lds_doubles() {
   return lds_doubles_of<T>(TNNZ);
 }
 
+// The run kernel's staging LDS (doubles): a phase's tile products, and an LP's
+// resident copy when it fits.  A 512-thread workgroup holds half of a CU's
+// waves (4 per SIMD under the register cap), so two share a CU and nothing
+// else fits beside them: each gets half of the CU's 160 KB less the kernel's
+// other LDS (under 3 KB).  Smaller workgroups keep the staging size, so that
+// as many of them fit a CU as its waves allow.
+constexpr int kRunLds512 = 9728;   // 76 KB
+template <class T>
+constexpr int run_lds_doubles_of(int block, int tnnz) {
+  return block == 512 && kRunLds512 > lds_doubles_of<T>(tnnz) ? kRunLds512
+                                                              : lds_doubles_of<T>(tnnz);
+}
+
 // Whether the run kernel of BLOCK threads (tile cap tnnz) holds an LP of this
 // shape in LDS: the rule it applies to each LP, for the host.
 template <class T>
-inline bool run_kernel_resident(int m, int n, long long nnz, int ntA, int ntT, int tnnz) {
-  return resident_fits<T>(m, n, nnz, ntA, ntT, 8LL * lds_doubles_of<T>(tnnz), tnnz);
+inline bool run_kernel_resident(int m, int n, long long nnz, int ntA, int ntT, int block,
+                                int tnnz) {
+  return resident_fits<T>(m, n, nnz, ntA, ntT, 8LL * run_lds_doubles_of<T>(block, tnnz), tnnz);
 }
 
 constexpr long long kNoStop = 0x7fffffffffffffffLL;
@@ -135,7 +149,7 @@ template <class T, int BLOCK, int TNNZ>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8)))
     pdlp_run_kernel(const LpDev<T>* __restrict__ lps, const StepCtl* __restrict__ ctl,
                     unsigned* done, int max_phases) {
-  __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
+  __shared__ __align__(16) double lds_raw[run_lds_doubles_of<T>(BLOCK, TNNZ)];
   __shared__ double red[(BLOCK / kWave + 2) * kMaxRed];
   __shared__ LpState ss;
   __shared__ Params sp;                  // the parameters, read once per launch

@@ -156,6 +156,7 @@ _SIGS = {
     "sovlp_server_info": (C.c_int, [P, C.c_int, C.POINTER(sovlp_info)]),
     "sovlp_server_outputs": (C.c_int, [P, C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
     "sovlp_server_refinement": (C.c_int, [P, C.c_int, C.POINTER(P)]),
+    "sovlp_server_history": (C.c_int, [P, C.c_int, C.POINTER(P)]),
     "sovlp_server_phase_counts": (C.c_int, [P, C.c_int, P, P]),
     "sovlp_server_release": (C.c_int, [P, C.c_int]),
     "sovlp_server_clock_hz": (C.c_int, [P, C.POINTER(F64)]),

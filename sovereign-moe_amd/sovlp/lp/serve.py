@@ -237,6 +237,13 @@ class DeviceServer:
             _lib.call("sovlp_server_refinement", self.h, int(slot), C.byref(r))
             ref = refinement_rows(view(r.value, 5 * int(inf.refine_rows)).cpu().numpy()
                                   .reshape(-1, 5), dlp.flip)
+        hist = []
+        nh = min(int(inf.history_len), self.history_cap)
+        if nh > 0:
+            hp = C.c_void_p()
+            _lib.call("sovlp_server_history", self.h, int(slot), C.byref(hp))
+            rows = view(hp.value, 5 * nh).cpu().numpy().reshape(-1, 5)
+            hist = [(int(r[0]), float(r[1]), float(r[2]), float(r[3]), float(r[4])) for r in rows]
         secs = inf.device_ticks / self.hz if self.hz else float("nan")
         st = Status(int(inf.status)) if inf.status >= 0 else Status.NOT_SOLVED
         info = PDLPInfo(iterations=int(inf.iterations), restarts=int(inf.restarts),
@@ -245,7 +252,7 @@ class DeviceServer:
                         dual_obj=inf.dual_obj, step_size=inf.step_size,
                         primal_weight=inf.primal_weight, scale_obj=inf.scale_obj,
                         device_seconds=secs, team="served", precision=self.precision,
-                        refinement=ref)
+                        refinement=ref, history=hist)
         sol = Solution(status=st, x=out[:n].copy(), objective=float(inf.objective),
                        y=out[n:n + m].copy(), reduced_costs=out[n + m:].copy(),
                        iterations=int(inf.iterations), time=secs, method="pdlp[gpu,served]")
