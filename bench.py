@@ -317,6 +317,23 @@ def traffic_key(lps: int, seed: int, precision: int, block: int, tuned: bool,
     return want
 
 
+SPMV_KEYS = ("op", "kernel", "precision", "rows", "cols", "nnz", "grid", "tiles")
+
+
+def spmv_traffic() -> dict:
+    """Counted traffic per launch of the SpMV section's products from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes over `bench.py --only spmv`,
+    keyed by the product's shape and launch (assembled by ``python -m
+    sovlp.lp.traffic``)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_spmv_tiled_200k_x_1M.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return {}
+    return {tuple(e[k] for k in SPMV_KEYS): float(e["traffic_bytes_per_launch"])
+            for e in d.get("products", [])}
+
+
 def measured_traffic(kernel: str, lps: int, seed: int, precision: int, block: int, tuned: bool,
                      refine: dict | None, replays: int, lds_matrix: bool = True):
     """HBM bytes per launch of ``kernel`` on this workload from the committed
@@ -635,9 +652,18 @@ def main():
     def spmv_section():
         from sovlp.spmv_bench import skewed_roofline
         res, _ = skewed_roofline(A=large().problem.A, panels=a.panels)
+        counted = spmv_traffic()
         for r in res:
+            t = counted.get(tuple(r[k] for k in SPMV_KEYS))
+            if t is not None:
+                # the committed counter passes' bytes per launch (below L2:
+                # Infinity Cache hits included) over this run's time
+                r["traffic_bytes"] = t
+                r["traffic_gbs"] = t / (r["ms"] * 1e-3) / 1e9
+                r["traffic_frac"] = r["traffic_gbs"] / HBM_PEAK_GBS
             log(f"spmv {r['op']} {r['kernel']} FP{r['precision']}: {r['ms'] * 1e3:.1f} us, "
-                f"{r['gbs']:.0f} GB/s ({r['frac']:.1%})")
+                f"{r['gbs']:.0f} GB/s ({r['frac']:.1%})"
+                + (f", counted {r['traffic_gbs']:.0f} GB/s" if t is not None else ""))
         return {"lp": "200000 x 1000000 skewed LP", "peak_gbs": HBM_PEAK_GBS,
                 "bytes_model": "nnz (s_val + 4) + (rows + 1) 4 + cols s_val + rows s_val",
                 "results": res}

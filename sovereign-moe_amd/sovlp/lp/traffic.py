@@ -123,7 +123,9 @@ def bytes_of_counts(counts: np.ndarray, sizes, precision: int) -> float:
 #
 # B is the JSON line of the profiled bench run, F / W the output directories of
 # its two counter passes (`rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`,
-# csv output), CF / CW the same two passes over `bench.py --only copy`.
+# csv output), CF / CW the same two passes over `bench.py --only copy`.  When B
+# is the line of `bench.py --only spmv`, the record holds each tiled product's
+# traffic per launch instead (spmv_record).
 # ---------------------------------------------------------------------------
 
 def counter_rows(path: str) -> list[dict]:
@@ -204,6 +206,52 @@ def traffic_record(bench_line: dict, fetch: str, write: str, copy_fetch: str, co
     }
 
 
+def dispatch_series(path: str, counter: str, match: str) -> list[float]:
+    """The counter's value per dispatch of the kernels named like ``match``, in
+    dispatch order."""
+    per: dict[int, float] = {}
+    for r in counter_rows(path):
+        if r.get("Counter_Name") != counter or match not in r["Kernel_Name"]:
+            continue
+        d = int(r.get("Dispatch_Id") or r.get("Correlation_Id"))
+        per[d] = per.get(d, 0.0) + float(r["Counter_Value"])
+    return [per[d] for d in sorted(per)]
+
+
+_SPMV_KERNEL = {64: "spmv_tiled_kernel<double", 32: "spmv_tiled_kernel<float"}
+SPMV_KEYS = ("op", "kernel", "precision", "rows", "cols", "nnz", "grid", "tiles")
+
+
+def spmv_record(section: dict, fetch: str, write: str, copy_fetch: str, copy_write: str,
+                copy_bytes: int = 256 << 20) -> dict:
+    """Counted traffic per launch of each tiled product of a ``bench.py --only
+    spmv`` run: the dispatches of each precision's kernel, in order, are the
+    products of that precision in the section's order, ``warmup + reps`` each;
+    the mean over each product's last ``reps`` dispatches, scaled by the copy
+    calibration."""
+    _, fs = copy_scale(copy_fetch, "FETCH_SIZE", copy_bytes)
+    _, ws = copy_scale(copy_write, "WRITE_SIZE", copy_bytes)
+    entries = []
+    for prec, match in _SPMV_KERNEL.items():
+        prods = [r for r in section["results"]
+                 if r["precision"] == prec and r["kernel"].startswith("tiled")]
+        f, w = dispatch_series(fetch, "FETCH_SIZE", match), dispatch_series(write, "WRITE_SIZE", match)
+        need = sum(r["warmup"] + r["reps"] for r in prods)
+        if len(f) != need or len(w) != need:
+            raise ValueError(f"{match}: {len(f)} / {len(w)} dispatches counted, {need} expected")
+        at = 0
+        for r in prods:
+            lo, hi = at + r["warmup"], at + r["warmup"] + r["reps"]
+            fk, wk = float(np.mean(f[lo:hi])), float(np.mean(w[lo:hi]))
+            e = {k: r[k] for k in SPMV_KEYS}
+            e.update(fetch_kib_per_launch=fk, write_kib_per_launch=wk,
+                     traffic_bytes_per_launch=(fk * fs + wk * ws) * 1024.0,
+                     algorithmic_bytes=r["bytes"])
+            entries.append(e)
+            at = hi
+    return {"lp": section["lp"], "copy_scale": {"fetch": fs, "write": ws}, "products": entries}
+
+
 def _main(argv=None) -> int:
     import argparse
     import json
@@ -214,12 +262,17 @@ def _main(argv=None) -> int:
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     line = json.loads(open(a.bench_json).read().strip().splitlines()[-1])
-    rec = traffic_record(line, a.fetch, a.write, a.copy_fetch, a.copy_write)
+    if "spmv" in line and "roofline" not in line:
+        rec = spmv_record(line["spmv"], a.fetch, a.write, a.copy_fetch, a.copy_write)
+        summary = [(e["op"], e["kernel"], e["precision"], e["traffic_bytes_per_launch"])
+                   for e in rec["products"]]
+    else:
+        rec = traffic_record(line, a.fetch, a.write, a.copy_fetch, a.copy_write)
+        summary = {k: rec[k] for k in ("traffic_bytes_per_launch", "algorithmic_bytes_per_launch")}
     with open(a.out, "w") as fh:
         json.dump(rec, fh, indent=1)
         fh.write("\n")
-    print(json.dumps({k: rec[k] for k in ("traffic_bytes_per_launch",
-                                          "algorithmic_bytes_per_launch")}))
+    print(json.dumps(summary))
     return 0
 
 

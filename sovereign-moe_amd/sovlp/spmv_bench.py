@@ -77,7 +77,8 @@ def measure(ptr: np.ndarray, idx: np.ndarray, val: np.ndarray, ncols: int, preci
     s_val = 8 if precision == 64 else 4
     b = algorithmic_bytes(rows, ncols, nnz, s_val)
     return dict(rows=rows, cols=ncols, nnz=nnz, precision=precision, grid=grid, tiles=nt,
-                ms=ms, bytes=b, gbs=b / (ms * 1e-3) / 1e9, frac=b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                ms=ms, bytes=b, gbs=b / (ms * 1e-3) / 1e9, frac=b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                warmup=warmup, reps=reps)
 
 
 def split_panels(ptr: np.ndarray, idx: np.ndarray, val: np.ndarray, ncols: int, panels: int):

@@ -241,3 +241,20 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(os.path.dirname(__file__), "..", "bench.py"),
                         "--gpus", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_bench_spmv_traffic_record_matches_the_section_keys():
+    """The committed SpMV counter record holds the eight tiled products of the
+    bench's spmv section, keyed as bench.py looks them up, with counted traffic
+    of at least the bytes of the matrix itself."""
+    import json
+    import bench
+    d = json.load(open(os.path.join(bench.ROOT, "profiles",
+                                    "pmc_traffic_spmv_tiled_200k_x_1M.json")))
+    rec = bench.spmv_traffic()
+    assert len(rec) == len(d["products"]) == 8
+    for e in d["products"]:
+        key = tuple(e[k] for k in bench.SPMV_KEYS)
+        assert rec[key] == e["traffic_bytes_per_launch"]
+        s_val = 8 if e["precision"] == 64 else 4
+        assert e["traffic_bytes_per_launch"] >= e["nnz"] * (s_val + 4)
