@@ -309,9 +309,11 @@ __device__ __forceinline__ void team_step(const LpDev<T>& L, int rank, int W, in
   }
 }
 
-// One LP over all gridDim.x workgroups.
+// One LP over all gridDim.x workgroups.  Registers are capped for 4 waves per
+// SIMD, as in the run kernel (the FP64 instance otherwise takes 132 and runs
+// 3 workgroups per CU).
 template <class T, int BLOCK, int TNNZ>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8)))
     pdlp_step_kernel(const LpDev<T>* __restrict__ lp, int parity, const StepCtl* __restrict__ ctl,
                      unsigned* done) {
   __shared__ __align__(16) double lds_raw[lds_doubles<T, TNNZ>()];
