@@ -206,3 +206,24 @@ def test_poisson_stream_with_warm_started_resolves():
         r = oracle.solve(p, eps_abs=1e-9, eps_rel=1e-9)
         assert r.status == 0
         assert abs(s.solution.objective - r.objective) <= 1e-6 * max(1.0, abs(r.objective))
+
+
+def test_served_history_matches_the_single_lp_engine():
+    """A served FP64 LP on one workgroup logs the same check history as the
+    single-LP engine on one workgroup (the same arithmetic): one row per check,
+    at iterations 64, 128, ..., read through sovlp_server_history."""
+    from sovlp.lp.pdlp import solve_pdlp
+    p = build_blend(BLEND["cases"][1])
+    srv = DeviceServer(2, _params(), precision=64, history_cap=4096)
+    try:
+        s = _solve_in(srv, [[(0, p, None)]])[0]
+    finally:
+        srv.close()
+    ref = solve_pdlp(p, PDLPParams(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6, device="gpu",
+                                   team="block"))
+    assert s.iterations == ref.iterations
+    h, r = s.info.history, ref.info.history
+    assert len(h) == len(r) > 0
+    assert [row[0] for row in h] == [64 * (i + 1) for i in range(len(h))]
+    for a, b in zip(h, r):
+        assert a == b
