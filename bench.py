@@ -759,7 +759,8 @@ def main():
     blocks = sorted(set(int(b) for b in eng.db.blocks))
     kernel = " + ".join(f"pdlp_run_kernel<{'double' if a.precision == 64 else 'float'}, {b}, "
                         f"{batch_geometry(a.precision, b)[1]}>" for b in blocks)
-    n_resident = int(eng.resident().sum())
+    modes = eng.lds_modes()
+    n_resident, n_vec = int((modes == 1).sum()), int((modes == 2).sum())
     eng.close()
     if fb is not None:
         fb.close()
@@ -788,7 +789,8 @@ def main():
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "optimal": opt,
-        "lds_resident_lps": n_resident,
+        "lds_resident_lps": n_resident,        # matrix and step vectors in LDS
+        "lds_vectors_only_lps": n_vec,         # only the step vectors in LDS
         "lps_per_s_all_statuses": a.lps * a.steps / wall,
         "max_rel_obj_err_vs_closed_form": float(max(rel)) if rel else None,   # optimal LPs
         "rel_obj_err_p99": float(np.percentile(rel, 99)) if rel else None,

@@ -261,10 +261,12 @@ int sovlp_spmv_panels_f32(int panels, int rows, const int32_t *ptrs, const int32
  *                _converged (pdlp.py:523-530) holds for X on the original.
  * flags          0, or SOVLP_FLAG_NO_LDS_MATRIX: the one-workgroup engines
  *                (batch, single LP on one workgroup, serving slots) then keep
- *                no LP's matrix in LDS.  By default an LP whose scaled matrix
- *                (both orientations, 16-bit indices), plans and step vectors
- *                fit the workgroup's staging LDS is held there for the PDHG
- *                loop; the results are the same bit for bit either way. */
+ *                no LP's matrix or step vectors in LDS.  By default an LP
+ *                whose scaled matrix (both orientations, 16-bit indices),
+ *                plans and step vectors fit the workgroup's staging LDS is
+ *                held there for the PDHG loop, and an LP whose matrix does
+ *                not fit keeps its step vectors there; the results are the
+ *                same bit for bit either way. */
 #define SOVLP_FLAG_NO_LDS_MATRIX 1
 typedef struct {
   double eps_abs, eps_rel, feas_cap;
@@ -401,11 +403,13 @@ typedef struct sovlp_batch sovlp_batch;
  * 256 or 512 threads per LP): the row cap and nonzero cap of a tile. */
 int sovlp_batch_geometry(int precision, int block, int *tile_rows, int *tile_nnz);
 
-/* Whether the one-workgroup engines on `block` threads hold an LP of this
- * shape (tiles_a / tiles_t: its tile counts for that block's geometry) in
- * LDS for the PDHG loop (see sovlp_params.flags): *resident = 1 or 0.  No
- * reference counterpart (the reference streams the matrix from memory in
- * every product, backend.py:118-152). */
+/* What the one-workgroup engines on `block` threads hold in LDS for the PDHG
+ * loop of an LP of this shape (tiles_a / tiles_t: its tile counts for that
+ * block's geometry; see sovlp_params.flags): *resident = 1 its scaled matrix
+ * and step vectors, 2 only its step vectors dx and dy (the products then
+ * gather from LDS, the matrix streams from memory), 0 nothing.  No reference
+ * counterpart (the reference streams the matrix from memory in every
+ * product, backend.py:118-152). */
 int sovlp_lds_resident(int precision, int block, int m, int n, int64_t nnz, int tiles_a,
                        int tiles_t, int *resident);
 

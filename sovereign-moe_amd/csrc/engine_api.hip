@@ -222,10 +222,9 @@ int sovlp_lds_resident(int precision, int block, int m, int n, int64_t nnz, int 
   if (!resident || m < 0 || n < 0 || nnz < 0 || tiles_a < 0 || tiles_t < 0)
     return fail(1, "bad arguments");
   const int tn = batch_tnnz(precision, block);
-  *resident = (precision == 64
-                   ? run_kernel_resident<double>(m, n, nnz, tiles_a, tiles_t, block, tn)
-                   : run_kernel_resident<float>(m, n, nnz, tiles_a, tiles_t, block, tn))
-                  ? 1 : 0;
+  *resident = precision == 64
+                  ? run_kernel_lds_mode<double>(m, n, nnz, tiles_a, tiles_t, block, tn)
+                  : run_kernel_lds_mode<float>(m, n, nnz, tiles_a, tiles_t, block, tn);
   return 0;
 }
 

@@ -39,12 +39,15 @@ constexpr int run_lds_doubles_of(int block, int tnnz) {
                                                               : lds_doubles_of<T>(tnnz);
 }
 
-// Whether the run kernel of BLOCK threads (tile cap tnnz) holds an LP of this
-// shape in LDS: the rule it applies to each LP, for the host.
+// What the run kernel of BLOCK threads (tile cap tnnz) holds in LDS for an LP
+// of this shape: 1 its matrix and step vectors, 2 only its step vectors, 0
+// nothing -- the rule it applies to each LP, for the host.
 template <class T>
-inline bool run_kernel_resident(int m, int n, long long nnz, int ntA, int ntT, int block,
-                                int tnnz) {
-  return resident_fits<T>(m, n, nnz, ntA, ntT, 8LL * run_lds_doubles_of<T>(block, tnnz), tnnz);
+inline int run_kernel_lds_mode(int m, int n, long long nnz, int ntA, int ntT, int block,
+                               int tnnz) {
+  const long long cap = 8LL * run_lds_doubles_of<T>(block, tnnz);
+  if (resident_fits<T>(m, n, nnz, ntA, ntT, cap, tnnz)) return 1;
+  return vec_fits<T>(m, n, cap, tnnz) ? 2 : 0;
 }
 
 constexpr long long kNoStop = 0x7fffffffffffffffLL;
@@ -173,9 +176,13 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
   const bool resident =
       uniform(!(sp.flags & kFlagNoLdsMatrix) &&
               resident_fits<T>(m, n, nnz, ntA, ntT, (long long)sizeof(lds_raw), TNNZ));
+  // else only dx and dy, above the loop phases' staging area
+  const bool vec = uniform(!resident && !(sp.flags & kFlagNoLdsMatrix) &&
+                           vec_fits<T>(m, n, (long long)sizeof(lds_raw), TNNZ));
   ResidentLds<T> rl{};
   if (resident) rl = carve_resident<T>(lds_raw, m, n, nnz, ntA);
-  bool res_ok = false;       // the resident copy holds the current matrix, dx and dy
+  if (vec) rl = carve_vec<T>(lds_raw, n, TNNZ);
+  bool res_ok = false;       // the LDS copy holds the current matrix (or only dx and dy)
   // thread 0: when the current phase started (the end of the previous one)
   long long tp = threadIdx.x == 0 ? (long long)wall_clock64() : 0;
   for (int it = 0; it < max_phases; ++it) {
@@ -187,15 +194,17 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
     // writes the state: thread 0 changes it right after the phase.
     const bool reduce = phase_reduces(ph, uniform(ss.next_mode));
     const bool on_res = resident && loop_phase(ph);
-    if (on_res && !res_ok) {
-      load_resident<T, BLOCK>(L, nnz, rl);
+    const bool on_vec = vec && loop_phase(ph);
+    if ((on_res || on_vec) && !res_ok) {
+      if (on_res) load_resident<T, BLOCK>(L, nnz, rl);
+      else load_vec<T, BLOCK>(L, rl);
       __syncthreads();
       res_ok = true;
     }
-    if (!on_res) res_ok = false;           // this phase may stage tiles over the copy
+    if (!on_res && !on_vec) res_ok = false;   // this phase may stage tiles over the copy
     double part[kMaxRed];
     init_part(ph, part);
-    PhaseWork<T, BLOCK, TNNZ> W(L, ss, reinterpret_cast<T*>(lds_raw), 0, 1, rl, on_res);
+    PhaseWork<T, BLOCK, TNNZ> W(L, ss, reinterpret_cast<T*>(lds_raw), 0, 1, rl, on_res, on_vec);
     W.run(ph, part);
     if (reduce) reduce_phase_to_lds<BLOCK>(ph, part, red);
     if (threadIdx.x == 0) {

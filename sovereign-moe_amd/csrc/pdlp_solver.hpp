@@ -226,6 +226,33 @@ __device__ inline ResidentLds<T> carve_resident(void* base, int m, int n, int nn
   return r;
 }
 
+// An LP whose matrix does not fit may still keep its two step vectors in LDS,
+// above the loop phases' staging area (TNNZ values of T): the products of the
+// staged tile pass then gather dx or dy from LDS (the matrix streams from
+// memory as before).  ResidentLds then holds only dx and dy.
+template <class T>
+__host__ __device__ inline long long vec_offset(int tnnz) {
+  return ((long long)tnnz * (long long)sizeof(T) + 15) / 16 * 16;
+}
+template <class T>
+__host__ __device__ inline bool vec_fits(int m, int n, long long cap, int tnnz) {
+  return vec_offset<T>(tnnz) + (long long)(m + n) * (long long)sizeof(T) <= cap;
+}
+template <class T>
+__device__ inline ResidentLds<T> carve_vec(void* base, int n, int tnnz) {
+  ResidentLds<T> r{};
+  r.dx = (SOV_LDS T*)((char*)base + vec_offset<T>(tnnz));
+  r.dy = r.dx + n;
+  return r;
+}
+
+// Every thread of the workgroup: copy the current dx and dy into LDS.
+template <class T, int BLOCK>
+__device__ inline void load_vec(const LpDev<T>& l, const ResidentLds<T>& r) {
+  for (int j = (int)threadIdx.x; j < l.n; j += BLOCK) r.dx[j] = l.dx[j];
+  for (int i = (int)threadIdx.x; i < l.m; i += BLOCK) r.dy[i] = l.dy[i];
+}
+
 // Every thread of the workgroup: fill the resident copy from global memory
 // (the scaled matrix, its plans, and the current dx and dy).  The caller
 // places a workgroup barrier before the copy is read.
@@ -292,15 +319,17 @@ struct PhaseWork {
   T* lds;
   double* ldsd;
   int rank, size;
-  // the LP's resident copy in LDS (loop phases) when has_res; held by value so
-  // that its pointers stay in registers
+  // the LP's resident copy in LDS (loop phases) when has_res, or only its dx
+  // and dy when has_vec; held by value so that its pointers stay in registers
   ResidentLds<T> res;
-  bool has_res;
+  bool has_res, has_vec, lds_dxdy;
 
   __device__ PhaseWork(const LpDev<T>& l_, const LpState& s_, T* lds_, int rank_, int size_,
-                       const ResidentLds<T>& res_ = ResidentLds<T>{}, bool has_res_ = false)
+                       const ResidentLds<T>& res_ = ResidentLds<T>{}, bool has_res_ = false,
+                       bool has_vec_ = false)
       : l(l_), s(s_), lds(lds_), ldsd(reinterpret_cast<double*>(lds_)), rank(rank_),
-        size(size_), res(res_), has_res(has_res_) {}
+        size(size_), res(res_), has_res(has_res_), has_vec(has_vec_),
+        lds_dxdy(has_res_ || has_vec_) {}
 
   __device__ __forceinline__ void erange(long long len, long long* a, long long* b) const {
     *a = len * rank / size;
@@ -352,7 +381,7 @@ struct PhaseWork {
     const T v = clampT<T>(xj - tt * (l.c[j] + aty), l.lo[j], l.hi[j]);
     const T d = v - xj;
     l.dx[j] = d;
-    if (has_res) res.dx[j] = d;
+    if (lds_dxdy) res.dx[j] = d;
     if (nm == NEXT_FUSED) {
       l.x[j] = v;
     } else {
@@ -559,7 +588,7 @@ struct PhaseWork {
           l.xn[j] = v;
           const T d = v - xj;
           l.dx[j] = d;
-          if (has_res) res.dx[j] = d;
+          if (lds_dxdy) res.dx[j] = d;
           part[0] += (double)d * (double)d;
         }
       } break;
@@ -577,15 +606,20 @@ struct PhaseWork {
           l.yn[i] = yv;
           const T d = yv - r.y;
           l.dy[i] = d;
-          if (has_res) res.dy[i] = d;
+          if (lds_dxdy) res.dy[i] = d;
           part[0] += (double)d * (double)a;
           part[1] += (double)d * (double)d;
         };
-        if (has_res)
+        if (has_res) {
           tile_pass_res<BLOCK, T>(res.valA, res.idxA, res.ptrA, res.tilesA, res.dx, a0, a1,
                                   pre, epi);
-        else
+        } else if (has_vec) {
+          auto prodv = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * res.dx[j]; };
+          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prodv, pre, epi,
+                                                 lds);
+        } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
+        }
       } break;
 
       case PH_B_ADAPT: {    // accepted trial
@@ -603,11 +637,16 @@ struct PhaseWork {
           l.sumx[j] = r.sumx + et * xj;
           next_primal(j, xj, aty, nm, tt, dxdx);
         };
-        if (has_res)
+        if (has_res) {
           tile_pass_res<BLOCK, T>(res.valT, res.idxT, res.ptrT, res.tilesT, res.dy, b0, b1,
                                   pre, epi);
-        else
+        } else if (has_vec) {
+          auto prodv = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * res.dy[i]; };
+          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prodv, pre, epi,
+                                                 lds);
+        } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
+        }
         for (long long i = i0 + tid; i < i1; i += BLOCK) {
           const T yv = l.yn[i];
           l.y[i] = yv;
@@ -630,15 +669,20 @@ struct PhaseWork {
           const T yv = dual_prox(w, r.q, sg, r.rl, r.ru);
           const T d = yv - r.y;
           l.dy[i] = d;
-          if (has_res) res.dy[i] = d;
+          if (lds_dxdy) res.dy[i] = d;
           l.y[i] = yv;
           l.sumy[i] = r.sumy + et * yv;
         };
-        if (has_res)
+        if (has_res) {
           tile_pass_res<BLOCK, T>(res.valA, res.idxA, res.ptrA, res.tilesA, res.dx, a0, a1,
                                   pre, epi);
-        else
+        } else if (has_vec) {
+          auto prodv = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * res.dx[j]; };
+          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prodv, pre, epi,
+                                                 lds);
+        } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
+        }
       } break;
 
       case PH_B_FUSED: {    // aty += A^T dy; sum_x += eta x (backend.py:282-291); next primal
@@ -665,7 +709,7 @@ struct PhaseWork {
           const T v = clampT<T>(xj - tt * (r.c + aty), r.lo, r.hi);
           const T d = v - xj;
           l.dx[j] = d;
-          if (has_res) res.dx[j] = d;
+          if (lds_dxdy) res.dx[j] = d;
           if (nm == NEXT_FUSED) {
             l.x[j] = v;
           } else {
@@ -673,11 +717,16 @@ struct PhaseWork {
             dxdx += (double)d * (double)d;
           }
         };
-        if (has_res)
+        if (has_res) {
           tile_pass_res<BLOCK, T>(res.valT, res.idxT, res.ptrT, res.tilesT, res.dy, b0, b1,
                                   pre, epi);
-        else
+        } else if (has_vec) {
+          auto prodv = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * res.dy[i]; };
+          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prodv, pre, epi,
+                                                 lds);
+        } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
+        }
         part[0] += dxdx;
       } break;
 

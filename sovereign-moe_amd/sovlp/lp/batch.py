@@ -30,7 +30,7 @@ from ..core.problem import ObjSense, Problem, Solution, Status
 from .pdlp import PDLPInfo, PDLPParams, plan_tiles, refinement_rows
 
 __all__ = ["BatchResult", "BatchEngine", "solve_batch", "DeviceBatch", "batch_geometry",
-           "lds_resident"]
+           "lds_mode", "lds_resident"]
 
 
 def phases_per_launch() -> int:
@@ -46,14 +46,21 @@ def batch_geometry(precision: int, block: int) -> tuple[int, int]:
     return r.value, t.value
 
 
-def lds_resident(precision: int, block: int, m: int, n: int, nnz: int, tiles_a: int,
-                 tiles_t: int) -> bool:
-    """Whether the run kernel on ``block`` threads holds an LP of this shape in
-    LDS for the PDHG loop (the device's own rule)."""
+def lds_mode(precision: int, block: int, m: int, n: int, nnz: int, tiles_a: int,
+             tiles_t: int) -> int:
+    """What the run kernel on ``block`` threads holds in LDS for the PDHG loop
+    of an LP of this shape (the device's own rule): 1 its matrix and step
+    vectors, 2 only its step vectors, 0 nothing."""
     r = C.c_int()
     _lib.call("sovlp_lds_resident", int(precision), int(block), int(m), int(n), int(nnz),
               int(tiles_a), int(tiles_t), C.byref(r))
-    return bool(r.value)
+    return int(r.value)
+
+
+def lds_resident(precision: int, block: int, m: int, n: int, nnz: int, tiles_a: int,
+                 tiles_t: int) -> bool:
+    """Whether the run kernel holds an LP of this shape, matrix and all, in LDS."""
+    return lds_mode(precision, block, m, n, nnz, tiles_a, tiles_t) == 1
 
 
 class DeviceBatch:
@@ -210,13 +217,19 @@ class BatchEngine:
         _lib.call("sovlp_batch_phase_ticks", self.h, out.ctypes.data, self.stream)
         return out
 
-    def resident(self) -> np.ndarray:
-        """Per LP: whether its matrix is held in LDS for the PDHG loop (always
-        False with ``PDLPParams(lds_matrix=False)``)."""
+    def lds_modes(self) -> np.ndarray:
+        """Per LP: what the run kernel holds in LDS for its PDHG loop, 1 its
+        matrix and step vectors, 2 only its step vectors, 0 nothing (always 0
+        with ``PDLPParams(lds_matrix=False)``)."""
         if not self.params.lds_matrix:
-            return np.zeros(self.count, dtype=bool)
-        return np.array([lds_resident(self.precision, int(b), m, n, nnz, ta, tt)
-                         for (m, n, nnz, ta, tt, _o, _f), b in zip(self.db.meta, self.db.blocks)])
+            return np.zeros(self.count, dtype=np.int32)
+        return np.array([lds_mode(self.precision, int(b), m, n, nnz, ta, tt)
+                         for (m, n, nnz, ta, tt, _o, _f), b in zip(self.db.meta, self.db.blocks)],
+                        dtype=np.int32)
+
+    def resident(self) -> np.ndarray:
+        """Per LP: whether its matrix is held in LDS for the PDHG loop."""
+        return self.lds_modes() == 1
 
     def sizes(self) -> list[tuple[int, int, int, int, int]]:
         """(m, n, nnz, tiles of A, tiles of A^T) of every LP."""

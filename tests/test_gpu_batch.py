@@ -125,9 +125,10 @@ def test_autotuned_mixed_configurations_in_one_batch():
 @pytest.mark.parametrize("precision", [32, 64])
 def test_lds_resident_matrix_is_bit_for_bit_the_streamed_one(precision):
     """The run kernel holds an LP's scaled matrix and step vectors in LDS when
-    they fit (the PDHG loop then reads no matrix from memory): every LP of a
-    mixed batch, resident or not, ends with the same iterations, x and y as with
-    every matrix streamed from memory (PDLPParams(lds_matrix=False)); the
+    they fit (the PDHG loop then reads no matrix from memory), and otherwise its
+    step vectors alone (the products gather from LDS): every LP of a mixed
+    batch, in either mode or neither, ends with the same iterations, x and y as
+    with everything streamed from memory (PDLPParams(lds_matrix=False)); the
     single-LP engine on one workgroup does the same."""
     from sovlp.lp.autotune import Autotuner, packaged_table
     from sovlp.lp.batch import BatchEngine
@@ -135,7 +136,7 @@ def test_lds_resident_matrix_is_bit_for_bit_the_streamed_one(precision):
     kos = _batch(48, base=7000)
     probs = [k.problem for k in kos]
     tuner = Autotuner.load(packaged_table(32)) if precision == 32 else None
-    out, resident = {}, None
+    out, modes = {}, None
     for flag in (True, False):
         par = PDLPParams(eps_abs=1e-6, eps_rel=1e-6, feas_cap=1e-6, device="gpu",
                          lds_matrix=flag)
@@ -144,11 +145,13 @@ def test_lds_resident_matrix_is_bit_for_bit_the_streamed_one(precision):
             eng.run()
             out[flag], _ = eng.solutions()
             if flag:
-                resident = eng.resident()
+                modes = eng.lds_modes()
         finally:
             eng.close()
-    print(f"[gpu] {int(resident.sum())} of {len(probs)} LPs resident", flush=True)
-    assert 0 < resident.sum() < len(probs)
+    resident = modes == 1
+    print(f"[gpu] {int(resident.sum())} of {len(probs)} LPs resident, "
+          f"{int((modes == 2).sum())} with only their step vectors in LDS", flush=True)
+    assert 0 < resident.sum() < len(probs) and (modes == 2).sum() > 0
     for i, (a, b) in enumerate(zip(out[True], out[False])):
         assert a.status == b.status and a.iterations == b.iterations, i
         assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y), i
