@@ -614,9 +614,8 @@ struct PhaseWork {
           tile_pass_res<BLOCK, T>(res.valA, res.idxA, res.ptrA, res.tilesA, res.dx, a0, a1,
                                   pre, epi);
         } else if (has_vec) {
-          auto prodv = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * res.dx[j]; };
-          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prodv, pre, epi,
-                                                 lds);
+          const LdsGather<T> ld{l.ai, l.sav, res.dx};
+          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.ap, l.tilesA, a0, a1, ld, pre, epi, lds);
         } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
         }
@@ -641,9 +640,8 @@ struct PhaseWork {
           tile_pass_res<BLOCK, T>(res.valT, res.idxT, res.ptrT, res.tilesT, res.dy, b0, b1,
                                   pre, epi);
         } else if (has_vec) {
-          auto prodv = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * res.dy[i]; };
-          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prodv, pre, epi,
-                                                 lds);
+          const LdsGather<T> ld{l.ti, l.stv, res.dy};
+          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.tp, l.tilesT, b0, b1, ld, pre, epi, lds);
         } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
         }
@@ -677,9 +675,8 @@ struct PhaseWork {
           tile_pass_res<BLOCK, T>(res.valA, res.idxA, res.ptrA, res.tilesA, res.dx, a0, a1,
                                   pre, epi);
         } else if (has_vec) {
-          auto prodv = [&](int q, int j, T (&v)[1]) { v[0] = l.sav[q] * res.dx[j]; };
-          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prodv, pre, epi,
-                                                 lds);
+          const LdsGather<T> ld{l.ai, l.sav, res.dx};
+          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.ap, l.tilesA, a0, a1, ld, pre, epi, lds);
         } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
         }
@@ -721,9 +718,8 @@ struct PhaseWork {
           tile_pass_res<BLOCK, T>(res.valT, res.idxT, res.ptrT, res.tilesT, res.dy, b0, b1,
                                   pre, epi);
         } else if (has_vec) {
-          auto prodv = [&](int q, int i, T (&v)[1]) { v[0] = l.stv[q] * res.dy[i]; };
-          tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prodv, pre, epi,
-                                                 lds);
+          const LdsGather<T> ld{l.ti, l.stv, res.dy};
+          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.tp, l.tilesT, b0, b1, ld, pre, epi, lds);
         } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
         }

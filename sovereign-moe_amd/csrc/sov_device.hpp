@@ -134,40 +134,71 @@ constexpr int kRowsPerLane = 4;
 // under the run kernel's 128-VGPR cap and is slower.)
 constexpr int kChunk = 4;
 
+// How a tile pass reads nonzero q: fetch(q) issues the loads that depend on no
+// other load, product(q, e, v) forms the NV products from what fetch loaded.
+//   IndexFetch  the column index; prod(q, j, v) loads the value and gathers
+//   LdsGather   the column index and the value together; the entry of the
+//               gathered vector comes from LDS (one memory latency per chunk
+//               of nonzeros, the gather an LDS read)
+template <class Prod>
+struct IndexFetch {
+  const int32_t* __restrict__ idx;
+  Prod& prod;
+  using E = int;
+  __device__ __forceinline__ E fetch(int q) const { return idx[q]; }
+  template <class V>
+  __device__ __forceinline__ void product(int q, E j, V& v) const { prod(q, j, v); }
+};
+
+template <class T>
+struct LdsGather {
+  const int32_t* __restrict__ idx;
+  const T* __restrict__ val;
+  const SOV_LDS T* __restrict__ g;
+  struct E {
+    int j;
+    T a;
+  };
+  __device__ __forceinline__ E fetch(int q) const { return E{idx[q], val[q]}; }
+  __device__ __forceinline__ void product(int, const E& e, T (&v)[1]) const {
+    v[0] = e.a * g[e.j];
+  }
+};
+
 // Stage the products of nonzeros [p0, p0 + nz) of a tile (nz <= TNNZ) into
 // LDS slot q = nonzero - p0.  Lanes past the end load the tile's last nonzero
 // (always valid) and discard it.
-template <int BLOCK, int TNNZ, int NV, class T, class Prod>
-__device__ __forceinline__ void stage_products(const int32_t* __restrict__ idx, int p0, int nz,
-                                               Prod& prod, T* lds) {
+template <int BLOCK, int TNNZ, int NV, class T, class Ld>
+__device__ __forceinline__ void stage_products(const Ld& ld, int p0, int nz, T* lds) {
   constexpr int PER = (TNNZ + BLOCK - 1) / BLOCK;
   constexpr int CH = PER < kChunk ? PER : kChunk;
+  using E = typename Ld::E;
   if (nz <= 0) return;
-  int jj[CH];
+  E jj[CH];
 #pragma unroll
   for (int k = 0; k < CH; ++k) {
     const int q = (int)threadIdx.x + k * BLOCK;
-    jj[k] = idx[p0 + (q < nz ? q : nz - 1)];
+    jj[k] = ld.fetch(p0 + (q < nz ? q : nz - 1));
   }
 #pragma unroll
   for (int c = 0; c < PER; c += CH) {
     if (c * BLOCK >= nz) break;          // nz is uniform: a scalar branch
     // the next chunk's column indices load while this chunk's products do:
     // one memory latency per chunk instead of two
-    int jn[CH];
+    E jn[CH];
     const bool next = c + CH < PER && (c + CH) * BLOCK < nz;
     if (next) {
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
         const int q = (int)threadIdx.x + (c + CH + k) * BLOCK;
-        jn[k] = idx[p0 + (q < nz ? q : nz - 1)];
+        jn[k] = ld.fetch(p0 + (q < nz ? q : nz - 1));
       }
     }
     T v[CH][NV];
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
       const int q = (int)threadIdx.x + (c + k) * BLOCK;
-      prod(p0 + (q < nz ? q : nz - 1), jj[k], v[k]);
+      ld.product(p0 + (q < nz ? q : nz - 1), jj[k], v[k]);
     }
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
@@ -199,11 +230,11 @@ __device__ __forceinline__ void stage_products(const int32_t* __restrict__ idx, 
 // runs the tiles one after another, so the number of dependent memory
 // latencies per tile, not bandwidth, sets the time of a small pass.
 // ------------------------------------------------------------------------
-template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Prod, class Pre, class Epi>
-__device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
-                                             const int32_t* __restrict__ idx,
+template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Ld, class Pre, class Epi>
+__device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr,
                                              const int32_t* __restrict__ tiles, int t0, int t1,
-                                             Prod& prod, Pre& pre, Epi& epi, T* lds) {
+                                             const Ld& ld, Pre& pre, Epi& epi, T* lds) {
+  using E = typename Ld::E;
   t0 = uniform(t0);
   t1 = uniform(t1);
   if (t0 >= t1) return;
@@ -224,28 +255,28 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
       // lane order q = p0 + tid, + BLOCK, ...; kChunk of them in flight at
       // once, and the next chunk's column indices load with this chunk's
       // products (one memory latency per chunk instead of two)
-      int jj[kChunk];
+      E jj[kChunk];
 #pragma unroll
       for (int c = 0; c < kChunk; ++c) {
         const int q = p0 + (int)threadIdx.x + c * BLOCK;
-        jj[c] = idx[q < p1 ? q : p1 - 1];
+        jj[c] = ld.fetch(q < p1 ? q : p1 - 1);
       }
       for (int base = p0; base < p1; base += kChunk * BLOCK) {
         const int nb = base + kChunk * BLOCK;
         const bool next = nb < p1;           // uniform: a scalar branch
-        int jn[kChunk];
+        E jn[kChunk];
         if (next) {
 #pragma unroll
           for (int c = 0; c < kChunk; ++c) {
             const int q = nb + (int)threadIdx.x + c * BLOCK;
-            jn[c] = idx[q < p1 ? q : p1 - 1];
+            jn[c] = ld.fetch(q < p1 ? q : p1 - 1);
           }
         }
         T v[kChunk][NV];
 #pragma unroll
         for (int c = 0; c < kChunk; ++c) {
           const int q = base + (int)threadIdx.x + c * BLOCK;
-          prod(q < p1 ? q : p1 - 1, jj[c], v[c]);
+          ld.product(q < p1 ? q : p1 - 1, jj[c], v[c]);
         }
 #pragma unroll
         for (int c = 0; c < kChunk; ++c) {
@@ -278,7 +309,7 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
         b[k] = ptr[rc + 1] - p0;
         pr[k] = pre(rc);
       }
-      stage_products<BLOCK, TNNZ, NV, T>(idx, p0, nz, prod, lds);
+      stage_products<BLOCK, TNNZ, NV, T>(ld, p0, nz, lds);
       __syncthreads();
       if (more) np1 = uniform(ptr[nr1]);
 #pragma unroll
@@ -319,7 +350,7 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
       const int rc = r0 + (active ? grp : rows - 1);
       const int a = ptr[rc] - p0, b = ptr[rc + 1] - p0;
       const auto pr = pre(rc);
-      stage_products<BLOCK, TNNZ, NV, T>(idx, p0, nz, prod, lds);
+      stage_products<BLOCK, TNNZ, NV, T>(ld, p0, nz, lds);
       __syncthreads();
       if (more) np1 = uniform(ptr[nr1]);
       T s[NV];
@@ -438,6 +469,16 @@ __device__ __forceinline__ void tile_pass_res(const SOV_LDS T* __restrict__ val,
       if (active && ln == 0) epi(r0 + grp, s, pr);
     }
   }
+}
+
+// The tile pass with the generic loader: column index, then prod(q, j, v).
+template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Prod, class Pre, class Epi>
+__device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
+                                             const int32_t* __restrict__ idx,
+                                             const int32_t* __restrict__ tiles, int t0, int t1,
+                                             Prod& prod, Pre& pre, Epi& epi, T* lds) {
+  const IndexFetch<Prod> ld{idx, prod};
+  tile_pass_ld<BLOCK, TNNZ, NV, MAXOP, T>(ptr, tiles, t0, t1, ld, pre, epi, lds);
 }
 
 // The tile pass without epilogue prefetch: epi(row, s[NV]).
