@@ -145,6 +145,7 @@ struct IndexFetch {
   const int32_t* __restrict__ idx;
   Prod& prod;
   using E = int;
+  static constexpr int kCh = kChunk;
   __device__ __forceinline__ E fetch(int q) const { return idx[q]; }
   template <class V>
   __device__ __forceinline__ void product(int q, E j, V& v) const { prod(q, j, v); }
@@ -159,6 +160,7 @@ struct LdsGather {
     int j;
     T a;
   };
+  static constexpr int kCh = 8;
   __device__ __forceinline__ E fetch(int q) const { return E{idx[q], val[q]}; }
   __device__ __forceinline__ void product(int, const E& e, T (&v)[1]) const {
     v[0] = e.a * g[e.j];
@@ -171,7 +173,7 @@ struct LdsGather {
 template <int BLOCK, int TNNZ, int NV, class T, class Ld>
 __device__ __forceinline__ void stage_products(const Ld& ld, int p0, int nz, T* lds) {
   constexpr int PER = (TNNZ + BLOCK - 1) / BLOCK;
-  constexpr int CH = PER < kChunk ? PER : kChunk;
+  constexpr int CH = PER < Ld::kCh ? PER : Ld::kCh;
   using E = typename Ld::E;
   if (nz <= 0) return;
   E jj[CH];
