@@ -313,7 +313,7 @@ def traffic_key(lps: int, seed: int, precision: int, block: int, tuned: bool,
         want += f", {refine['rounds']} rounds at {refine['round_eps']:g}, stall " \
                 f"{refine['stall_open']}/{refine['stall']}"
     if lds_matrix:
-        want += ", matrices in LDS where they fit"
+        want += ", matrices or step vectors in LDS where they fit"
     return want
 
 
