@@ -126,6 +126,47 @@ __device__ __forceinline__ void block_combine(T (&s)[NV], T* scratch) {
 // BLOCK rows (the host plans tiles with that row cap).
 constexpr int kRowsPerLane = 4;
 
+// Fold the staged products lds[u * TNNZ + q], q = a, a + step, ... < b, into
+// s[u] (sum, or max if MAXOP) in that order.  Four LDS reads are issued before
+// the first of them is used, so a row costs about len / 4 LDS round trips
+// instead of len; the operations and their order are those of a loop over
+// one product at a time, so the result is the same bit for bit.  The last
+// 1-3 products read in-range slots (clamped to b - 1) and use only their own.
+template <int TNNZ, int NV, bool MAXOP, class T>
+__device__ __forceinline__ void staged_row_fold(const T* lds, int a, int b, int step,
+                                                T (&s)[NV]) {
+  int q = a;
+  for (; q + 3 * step < b; q += 4 * step) {
+    T v[4][NV];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int u = 0; u < NV; ++u) v[r][u] = lds[u * TNNZ + q + r * step];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int u = 0; u < NV; ++u) s[u] = MAXOP ? tmax(s[u], v[r][u]) : s[u] + v[r][u];
+    }
+  }
+  if (q < b) {
+    T v[3][NV];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int qr = q + r * step < b ? q + r * step : b - 1;
+#pragma unroll
+      for (int u = 0; u < NV; ++u) v[r][u] = lds[u * TNNZ + qr];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      if (q + r * step < b) {
+#pragma unroll
+        for (int u = 0; u < NV; ++u) s[u] = MAXOP ? tmax(s[u], v[r][u]) : s[u] + v[r][u];
+      }
+    }
+  }
+}
+
 // Nonzeros per lane whose loads are issued together: the column indices of a
 // chunk, then its products (values and gathered entries), then the LDS
 // stores; the next chunk's indices are loaded with this chunk's products.  A
@@ -321,10 +362,7 @@ __device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr,
           T s[NV];
 #pragma unroll
           for (int u = 0; u < NV; ++u) s[u] = T(0);
-          for (int q = a[k]; q < b[k]; ++q) {
-#pragma unroll
-            for (int u = 0; u < NV; ++u) s[u] = MAXOP ? tmax(s[u], lds[u * TNNZ + q]) : s[u] + lds[u * TNNZ + q];
-          }
+          staged_row_fold<TNNZ, NV, MAXOP, T>(lds, a[k], b[k], 1, s);
           epi(r0 + row, s, pr[k]);
         }
       }
@@ -335,10 +373,7 @@ __device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr,
         T s[NV];
 #pragma unroll
         for (int u = 0; u < NV; ++u) s[u] = T(0);
-        for (int q = ra; q < rb; ++q) {
-#pragma unroll
-          for (int u = 0; u < NV; ++u) s[u] = MAXOP ? tmax(s[u], lds[u * TNNZ + q]) : s[u] + lds[u * TNNZ + q];
-        }
+        staged_row_fold<TNNZ, NV, MAXOP, T>(lds, ra, rb, 1, s);
         epi(r0 + row, s, pq);
       }
       __syncthreads();
@@ -358,12 +393,7 @@ __device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr,
       T s[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k) s[k] = T(0);
-      if (active) {
-        for (int q = a + ln; q < b; q += tpr) {
-#pragma unroll
-          for (int k = 0; k < NV; ++k) s[k] = MAXOP ? tmax(s[k], lds[k * TNNZ + q]) : s[k] + lds[k * TNNZ + q];
-        }
-      }
+      if (active) staged_row_fold<TNNZ, NV, MAXOP, T>(lds, a + ln, b, tpr, s);
       for (int o = tpr >> 1; o > 0; o >>= 1) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
