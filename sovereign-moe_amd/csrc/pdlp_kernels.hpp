@@ -103,7 +103,7 @@ __device__ __forceinline__ void take_transition(const LpDev<T>& L, const LpState
   if ((ph == PH_B_ADAPT || ph == PH_B_FUSED) && s.next_mode != NEXT_NONE)
     L.pcount[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
   Transition tr{ctl->P, L.offset0, L.flip};
-  ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap, L.rhist);
+  ss.phase = tr.next(ss, ph, t, L.hist, L.hist_cap, L.rhist);
   if (ss.phase == PH_DONE) {
     ss.ticks = (long long)wall_clock64() - ss.tick0;
     atomicAdd(done, 1u);
@@ -125,7 +125,7 @@ __device__ __forceinline__ void transition_in_lds(const LpDev<T>& L, int ph, con
   if ((ph == PH_B_ADAPT || ph == PH_B_FUSED) && ss.next_mode != NEXT_NONE)
     pc[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
   Transition tr{P, L.offset0, L.flip};
-  ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap, L.rhist);
+  ss.phase = tr.next(ss, ph, t, L.hist, L.hist_cap, L.rhist);
   if (ss.phase == PH_DONE) {
     ss.ticks = (long long)wall_clock64() - ss.tick0;
     atomicAdd(done, 1u);
@@ -283,7 +283,7 @@ __device__ __forceinline__ void team_step(const LpDev<T>& L, int rank, int W, in
           L.pcount[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
       }
       Transition tr{ctl->P, L.offset0, L.flip, lead};
-      ss.phase = tr.apply(ss, ph, t, L.hist, L.hist_cap, L.rhist);
+      ss.phase = tr.next(ss, ph, t, L.hist, L.hist_cap, L.rhist);
       ss.pending = 0;
       if (ss.phase == PH_DONE && lead) {
         ss.ticks = (long long)wall_clock64() - ss.tick0;

@@ -1014,9 +1014,17 @@ struct Transition {
 
   __device__ __forceinline__ bool refining() const { return P.refine_rounds > 0; }
 
+  // k mod a period (k >= 0, period >= 1): 32-bit division while k fits in 32
+  // bits (a 64-bit one is a long instruction sequence on the thread that
+  // takes the transition), the same value either way
+  __device__ __forceinline__ static long long kmod(long long k, int period) {
+    return (unsigned long long)k < 0x100000000ULL ? (long long)((unsigned)k % (unsigned)period)
+                                                 : k % period;
+  }
+
   __device__ __forceinline__ int next_mode(const LpState& s, long long k1) const {
-    if (k1 % P.check_every == 0 || k1 >= s.kmax) return NEXT_NONE;
-    return (k1 % P.stepsize_every == 0) ? NEXT_TRIAL : NEXT_FUSED;
+    if (kmod(k1, P.check_every) == 0 || k1 >= s.kmax) return NEXT_NONE;
+    return kmod(k1, P.stepsize_every) == 0 ? NEXT_TRIAL : NEXT_FUSED;
   }
 
   // top of the loop body (pdlp.py:313-317, 335): limits, then the step flavour
@@ -1026,11 +1034,11 @@ struct Transition {
       s.status = ST_ITERATION_LIMIT;
       return PH_FINAL;
     }
-    if (s.k % P.check_every == 0 && s.time_flag) {
+    if (kmod(s.k, P.check_every) == 0 && s.time_flag) {
       s.status = ST_TIME_LIMIT;
       return PH_FINAL;
     }
-    if (s.k % P.stepsize_every == 0) {
+    if (kmod(s.k, P.stepsize_every) == 0) {
       s.trial = 0;
       ++s.attempts;
       if (s.primal_ready) {
@@ -1051,8 +1059,17 @@ struct Transition {
     ++s.epoch;
     s.primal_ready = s.next_mode != NEXT_NONE;
     if (s.next_mode == NEXT_TRIAL) s.dxdx = dxdx;
-    if (s.k % P.check_every == 0) return PH_CHECK;
+    if (kmod(s.k, P.check_every) == 0) return PH_CHECK;
     return loop_top(s);
+  }
+
+  // The transition of phase ph, by one thread.  The end of an iteration (after
+  // a B phase), the most frequent transition, is taken inline on the caller's
+  // state (in LDS: plain LDS accesses, no call); every other one out of line.
+  __device__ __forceinline__ int next(LpState& s, int ph, const double* t, double* hist,
+                                      long long hist_cap, double* rhist) const {
+    if (ph == PH_B_FUSED || ph == PH_B_ADAPT) return end_iteration(s, t[0]);
+    return apply(s, ph, t, hist, hist_cap, rhist);
   }
 
   // Called by one thread with `s` and `t` in LDS (out of line: the scalar logic
