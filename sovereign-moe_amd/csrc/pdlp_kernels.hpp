@@ -80,6 +80,15 @@ __device__ __forceinline__ void reduce_phase_to_lds(int ph, const double (&part)
   const uint32_t mm = phase_maxmask(ph);
   if (ph == PH_CHECK) {
     block_reduce_to_lds<BLOCK, kMaxRed>(part, mm, red);
+  } else if (ph == PH_P_TRIAL || ph == PH_B_ADAPT || ph == PH_B_FUSED) {
+    // the loop's transitions read one total (||dx||^2) ...
+    double v[1] = {part[0]};
+    block_reduce_to_lds<BLOCK, 1>(v, mm, red);
+  } else if (ph == PH_A_TRIAL) {
+    // ... or two (dy.adx, ||dy||^2): each total is reduced on its own, so
+    // leaving the unread ones out changes none of the others
+    double v[2] = {part[0], part[1]};
+    block_reduce_to_lds<BLOCK, 2>(v, mm, red);
   } else {
     double v[6];
 #pragma unroll
@@ -130,6 +139,18 @@ __device__ __forceinline__ void transition_in_lds(const LpDev<T>& L, int ph, con
     ss.ticks = (long long)wall_clock64() - ss.tick0;
     atomicAdd(done, 1u);
   }
+}
+
+// Thread 0: the end of an iteration (after a B phase), the most frequent
+// transition, on the state and parameters in LDS: LDS accesses only (the LP's
+// offset and sense, which only the other transitions read, are not loaded).
+// It never ends the solve.
+__device__ __forceinline__ void end_iteration_in_lds(int ph, const double* t, const Params& P,
+                                                     int* pc, LpState& ss) {
+  pc[ph] += 1;
+  if (ss.next_mode != NEXT_NONE) pc[ph == PH_B_ADAPT ? kSlotBAdaptNext : kSlotBFusedNext] += 1;
+  const Transition tr{P, 0.0, 0};
+  ss.phase = tr.end_iteration(ss, t[0]);
 }
 
 // The phases of the PDHG loop: they read the matrix only through tile passes
@@ -211,6 +232,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
       if (ph == PH_A_FUSED) {
         pc[PH_A_FUSED] += 1;
         ss.phase = PH_B_FUSED;
+      } else if (ph == PH_B_FUSED || ph == PH_B_ADAPT) {
+        end_iteration_in_lds(ph, red + (BLOCK / kWave) * kMaxRed, sp, pc, ss);
+      } else if (ph == PH_P_TRIAL) {     // the P_TRIAL case of Transition::apply
+        pc[PH_P_TRIAL] += 1;
+        ss.dxdx = red[(BLOCK / kWave) * kMaxRed];
+        ss.phase = PH_A_TRIAL;
       } else {
         transition_in_lds(L, ph, red + (BLOCK / kWave) * kMaxRed, ctl, sp, done, pc, ss);
       }

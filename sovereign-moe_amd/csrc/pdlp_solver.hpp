@@ -1014,12 +1014,13 @@ struct Transition {
 
   __device__ __forceinline__ bool refining() const { return P.refine_rounds > 0; }
 
-  // k mod a period (k >= 0, period >= 1): 32-bit division while k fits in 32
-  // bits (a 64-bit one is a long instruction sequence on the thread that
-  // takes the transition), the same value either way
+  // k mod a period (period >= 1): a mask when the period is a power of two
+  // (the defaults, 64 and 8) and k >= 0, else the division, which is a long
+  // instruction sequence on the one thread that takes the transition; the
+  // same value either way
   __device__ __forceinline__ static long long kmod(long long k, int period) {
-    return (unsigned long long)k < 0x100000000ULL ? (long long)((unsigned)k % (unsigned)period)
-                                                 : k % period;
+    if (k >= 0 && (period & (period - 1)) == 0) return k & (long long)(period - 1);
+    return k % period;
   }
 
   __device__ __forceinline__ int next_mode(const LpState& s, long long k1) const {
