@@ -41,16 +41,48 @@ __device__ __forceinline__ double nanmax(double a, double b) {
   return (a != a) ? a : ((b != b) ? b : (a > b ? a : b));
 }
 
-template <class T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+// The 32 bits of lane (l ^ O), O a power of two below 64, by the cheapest
+// exchange for that distance: DPP within quads (1: quad_perm [1,0,3,2], 2:
+// [2,3,0,1]) and rows of 16 (8: row_ror:8), ds_swizzle's bitmask mode within
+// halves of 32 (4, 16: xor mask in bits 14:10, and mask 0x1f), ds_bpermute
+// across the halves (32).  Every lane of the wave must be active.
+template <int O>
+__device__ __forceinline__ int xor_lane32(int x) {
+  if constexpr (O == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+  else if constexpr (O == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+  else if constexpr (O == 8) return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);
+  else if constexpr (O == 4) return __builtin_amdgcn_ds_swizzle(x, (4 << 10) | 0x1F);
+  else if constexpr (O == 16) return __builtin_amdgcn_ds_swizzle(x, (16 << 10) | 0x1F);
+  else return __shfl_xor(x, O);
+}
+template <int O>
+__device__ __forceinline__ double xor_lane(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)xor_lane32<O>((int)(unsigned)b);
+  const unsigned hi = (unsigned)xor_lane32<O>((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Butterfly over the wave, partners at distance 32, 16, ..., 1 in that order
+// (each lane adds its partner's value to its own): every lane ends with the
+// same total, formed in the same order whichever exchange moves the bits.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += xor_lane<32>(v);
+  v += xor_lane<16>(v);
+  v += xor_lane<8>(v);
+  v += xor_lane<4>(v);
+  v += xor_lane<2>(v);
+  v += xor_lane<1>(v);
   return v;
 }
 
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o));
+  v = nanmax(v, xor_lane<32>(v));
+  v = nanmax(v, xor_lane<16>(v));
+  v = nanmax(v, xor_lane<8>(v));
+  v = nanmax(v, xor_lane<4>(v));
+  v = nanmax(v, xor_lane<2>(v));
+  v = nanmax(v, xor_lane<1>(v));
   return v;
 }
 
