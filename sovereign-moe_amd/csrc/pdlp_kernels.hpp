@@ -47,7 +47,7 @@ inline int run_kernel_lds_mode(int m, int n, long long nnz, int ntA, int ntT, in
                                int tnnz) {
   const long long cap = 8LL * run_lds_doubles_of<T>(block, tnnz);
   if (resident_fits<T>(m, n, nnz, ntA, ntT, cap, tnnz)) return 1;
-  return vec_fits<T>(m, n, cap, tnnz) ? 2 : 0;
+  return vec_fits<T>(m, n, ntA, ntT, cap, tnnz) ? 2 : 0;
 }
 
 constexpr long long kNoStop = 0x7fffffffffffffffLL;
@@ -199,10 +199,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
               resident_fits<T>(m, n, nnz, ntA, ntT, (long long)sizeof(lds_raw), TNNZ));
   // else only dx and dy, above the loop phases' staging area
   const bool vec = uniform(!resident && !(sp.flags & kFlagNoLdsMatrix) &&
-                           vec_fits<T>(m, n, (long long)sizeof(lds_raw), TNNZ));
+                           vec_fits<T>(m, n, ntA, ntT, (long long)sizeof(lds_raw), TNNZ));
   ResidentLds<T> rl{};
   if (resident) rl = carve_resident<T>(lds_raw, m, n, nnz, ntA);
-  if (vec) rl = carve_vec<T>(lds_raw, n, TNNZ);
+  if (vec) rl = carve_vec<T>(lds_raw, m, n, ntA, ntT, TNNZ);
   bool res_ok = false;       // the LDS copy holds the current matrix (or only dx and dy)
   // thread 0: when the current phase started (the end of the previous one)
   long long tp = threadIdx.x == 0 ? (long long)wall_clock64() : 0;

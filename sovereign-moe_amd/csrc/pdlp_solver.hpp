@@ -188,6 +188,10 @@ template <class T>
 struct ResidentLds {
   SOV_LDS T *valA, *valT, *dx, *dy;
   SOV_LDS uint16_t *idxA, *idxT, *ptrA, *ptrT, *tilesA, *tilesT;
+  // only dx, dy in LDS: the tile bounds of both orientations (first row and
+  // first nonzero of each tile, and the ends), so the streamed passes find
+  // their tiles without dependent loads from memory
+  SOV_LDS int32_t *rowsA, *nzsA, *rowsT, *nzsT;
 };
 
 // LDS bytes of an LP's resident copy
@@ -235,14 +239,22 @@ __host__ __device__ inline long long vec_offset(int tnnz) {
   return ((long long)tnnz * (long long)sizeof(T) + 15) / 16 * 16;
 }
 template <class T>
-__host__ __device__ inline bool vec_fits(int m, int n, long long cap, int tnnz) {
-  return vec_offset<T>(tnnz) + (long long)(m + n) * (long long)sizeof(T) <= cap;
+__host__ __device__ inline bool vec_fits(int m, int n, int ntA, int ntT, long long cap,
+                                         int tnnz) {
+  return vec_offset<T>(tnnz) + (long long)(m + n) * (long long)sizeof(T) +
+             8LL * (ntA + ntT + 2) <=
+         cap;
 }
 template <class T>
-__device__ inline ResidentLds<T> carve_vec(void* base, int n, int tnnz) {
+__device__ inline ResidentLds<T> carve_vec(void* base, int m, int n, int ntA, int ntT,
+                                           int tnnz) {
   ResidentLds<T> r{};
   r.dx = (SOV_LDS T*)((char*)base + vec_offset<T>(tnnz));
   r.dy = r.dx + n;
+  r.rowsA = (SOV_LDS int32_t*)(r.dy + m);
+  r.nzsA = r.rowsA + (ntA + 1);
+  r.rowsT = r.nzsA + (ntA + 1);
+  r.nzsT = r.rowsT + (ntT + 1);
   return r;
 }
 
@@ -251,6 +263,16 @@ template <class T, int BLOCK>
 __device__ inline void load_vec(const LpDev<T>& l, const ResidentLds<T>& r) {
   for (int j = (int)threadIdx.x; j < l.n; j += BLOCK) r.dx[j] = l.dx[j];
   for (int i = (int)threadIdx.x; i < l.m; i += BLOCK) r.dy[i] = l.dy[i];
+  for (int t = (int)threadIdx.x; t <= l.ntA; t += BLOCK) {
+    const int row = l.tilesA[t];
+    r.rowsA[t] = row;
+    r.nzsA[t] = l.ap[row];
+  }
+  for (int t = (int)threadIdx.x; t <= l.ntT; t += BLOCK) {
+    const int row = l.tilesT[t];
+    r.rowsT[t] = row;
+    r.nzsT[t] = l.tp[row];
+  }
 }
 
 // Every thread of the workgroup: fill the resident copy from global memory
@@ -615,7 +637,8 @@ struct PhaseWork {
                                   pre, epi);
         } else if (has_vec) {
           const LdsGather<T> ld{l.ai, l.sav, res.dx};
-          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.ap, l.tilesA, a0, a1, ld, pre, epi, lds);
+          const LdsBounds bd{res.rowsA, res.nzsA};
+          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.ap, bd, a0, a1, ld, pre, epi, lds);
         } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
         }
@@ -641,7 +664,8 @@ struct PhaseWork {
                                   pre, epi);
         } else if (has_vec) {
           const LdsGather<T> ld{l.ti, l.stv, res.dy};
-          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.tp, l.tilesT, b0, b1, ld, pre, epi, lds);
+          const LdsBounds bd{res.rowsT, res.nzsT};
+          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.tp, bd, b0, b1, ld, pre, epi, lds);
         } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
         }
@@ -676,7 +700,8 @@ struct PhaseWork {
                                   pre, epi);
         } else if (has_vec) {
           const LdsGather<T> ld{l.ai, l.sav, res.dx};
-          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.ap, l.tilesA, a0, a1, ld, pre, epi, lds);
+          const LdsBounds bd{res.rowsA, res.nzsA};
+          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.ap, bd, a0, a1, ld, pre, epi, lds);
         } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.ap, l.ai, l.tilesA, a0, a1, prod, pre, epi, lds);
         }
@@ -719,7 +744,8 @@ struct PhaseWork {
                                   pre, epi);
         } else if (has_vec) {
           const LdsGather<T> ld{l.ti, l.stv, res.dy};
-          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.tp, l.tilesT, b0, b1, ld, pre, epi, lds);
+          const LdsBounds bd{res.rowsT, res.nzsT};
+          tile_pass_ld<BLOCK, TNNZ, 1, false, T>(l.tp, bd, b0, b1, ld, pre, epi, lds);
         } else {
           tile_pass_pf<BLOCK, TNNZ, 1, false, T>(l.tp, l.ti, l.tilesT, b0, b1, prod, pre, epi, lds);
         }

@@ -290,6 +290,23 @@ __device__ __forceinline__ void stage_products(const Ld& ld, int p0, int nz, T* 
   }
 }
 
+// Where a tile pass finds its tiles: tile t covers rows [row(t), row(t+1)) and
+// nonzeros [nz(t), nz(t+1)), nz(t) = ptr[row(t)].  GlobalBounds reads the
+// plan and the row pointers in memory (two dependent loads at the start of a
+// pass and one per tile after it); LdsBounds reads both from a copy in LDS.
+struct GlobalBounds {
+  const int32_t* __restrict__ ptr;
+  const int32_t* __restrict__ tiles;
+  __device__ __forceinline__ int row(int t) const { return uniform(tiles[t]); }
+  __device__ __forceinline__ int nz(int, int r) const { return uniform(ptr[r]); }
+};
+struct LdsBounds {
+  const SOV_LDS int32_t* rows;
+  const SOV_LDS int32_t* nzs;
+  __device__ __forceinline__ int row(int t) const { return uniform(rows[t]); }
+  __device__ __forceinline__ int nz(int t, int) const { return uniform(nzs[t]); }
+};
+
 // ------------------------------------------------------------------------
 // CSR-Adaptive tile pass.
 //   ptr   : row pointers (LP-local int32 offsets), rows+1 entries
@@ -305,19 +322,20 @@ __device__ __forceinline__ void stage_products(const Ld& ld, int p0, int nz, T* 
 // runs the tiles one after another, so the number of dependent memory
 // latencies per tile, not bandwidth, sets the time of a small pass.
 // ------------------------------------------------------------------------
-template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Ld, class Pre, class Epi>
-__device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr,
-                                             const int32_t* __restrict__ tiles, int t0, int t1,
-                                             const Ld& ld, Pre& pre, Epi& epi, T* lds) {
+template <int BLOCK, int TNNZ, int NV, bool MAXOP, class T, class Bd, class Ld, class Pre,
+          class Epi>
+__device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr, const Bd& bd,
+                                             int t0, int t1, const Ld& ld, Pre& pre, Epi& epi,
+                                             T* lds) {
   using E = typename Ld::E;
   t0 = uniform(t0);
   t1 = uniform(t1);
   if (t0 >= t1) return;
-  int r0 = uniform(tiles[t0]), r1 = uniform(tiles[t0 + 1]);
-  int p0 = uniform(ptr[r0]), p1 = uniform(ptr[r1]);
+  int r0 = bd.row(t0), r1 = bd.row(t0 + 1);
+  int p0 = bd.nz(t0, r0), p1 = bd.nz(t0 + 1, r1);
   for (int t = t0; t < t1; ++t) {
     const bool more = t + 1 < t1;
-    const int nr1 = more ? uniform(tiles[t + 2]) : r1;   // next tile's end row
+    const int nr1 = more ? bd.row(t + 2) : r1;   // next tile's end row
     const int nz = p1 - p0, rows = r1 - r0;
     int np1 = p1;
     if (nz > TNNZ) {
@@ -366,7 +384,7 @@ __device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr,
           for (int c = 0; c < kChunk; ++c) jj[c] = jn[c];
         }
       }
-      if (more) np1 = uniform(ptr[nr1]);
+      if (more) np1 = bd.nz(t + 2, nr1);
       block_combine<BLOCK, NV, MAXOP, T>(s, lds);
       if (threadIdx.x == 0) epi(r0, s, pr);
       __syncthreads();
@@ -386,7 +404,7 @@ __device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr,
       }
       stage_products<BLOCK, TNNZ, NV, T>(ld, p0, nz, lds);
       __syncthreads();
-      if (more) np1 = uniform(ptr[nr1]);
+      if (more) np1 = bd.nz(t + 2, nr1);
 #pragma unroll
       for (int k = 0; k < kRowsPerLane; ++k) {
         const int row = (int)threadIdx.x + k * BLOCK;
@@ -421,7 +439,7 @@ __device__ __forceinline__ void tile_pass_ld(const int32_t* __restrict__ ptr,
       const auto pr = pre(rc);
       stage_products<BLOCK, TNNZ, NV, T>(ld, p0, nz, lds);
       __syncthreads();
-      if (more) np1 = uniform(ptr[nr1]);
+      if (more) np1 = bd.nz(t + 2, nr1);
       T s[NV];
 #pragma unroll
       for (int k = 0; k < NV; ++k) s[k] = T(0);
@@ -542,7 +560,8 @@ __device__ __forceinline__ void tile_pass_pf(const int32_t* __restrict__ ptr,
                                              const int32_t* __restrict__ tiles, int t0, int t1,
                                              Prod& prod, Pre& pre, Epi& epi, T* lds) {
   const IndexFetch<Prod> ld{idx, prod};
-  tile_pass_ld<BLOCK, TNNZ, NV, MAXOP, T>(ptr, tiles, t0, t1, ld, pre, epi, lds);
+  const GlobalBounds bd{ptr, tiles};
+  tile_pass_ld<BLOCK, TNNZ, NV, MAXOP, T>(ptr, bd, t0, t1, ld, pre, epi, lds);
 }
 
 // The tile pass without epilogue prefetch: epi(row, s[NV]).
