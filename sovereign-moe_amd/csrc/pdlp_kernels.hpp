@@ -238,6 +238,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8
         pc[PH_P_TRIAL] += 1;
         ss.dxdx = red[(BLOCK / kWave) * kMaxRed];
         ss.phase = PH_A_TRIAL;
+      } else if (ph == PH_A_TRIAL) {     // the step-size rule on the LDS state
+        pc[PH_A_TRIAL] += 1;
+        const Transition tr{sp, 0.0, 0};
+        ss.phase = tr.trial_step(ss, red + (BLOCK / kWave) * kMaxRed);
       } else {
         transition_in_lds(L, ph, red + (BLOCK / kWave) * kMaxRed, ctl, sp, done, pc, ss);
       }

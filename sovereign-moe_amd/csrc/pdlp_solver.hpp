@@ -1020,6 +1020,10 @@ struct PhaseWork {
   }
 };
 
+// pow out of line: the step-size rule is taken inline in the run kernel, and
+// pow's code there would cost the phases registers
+__device__ __noinline__ double pow_call(double x, double y) { return pow(x, y); }
+
 // ---------------------------------------------------------------------------
 // Transition: applies the reduced values of phase `ph` to the state and picks
 // the next phase.  Scalar logic only; runs on one thread.
@@ -1077,6 +1081,36 @@ struct Transition {
     s.primal_ready = 0;
     s.next_mode = next_mode(s, s.k + 1);
     return PH_A_FUSED;
+  }
+
+  // after the trial's A phase: the adaptive step-size rule (accept, or
+  // shrink and try again), t = (dy.adx, ||dy||^2)
+  __device__ __forceinline__ int trial_step(LpState& s, const double* t) const {
+    const double interaction = fabs(t[0]);
+    const double movement = 0.5 * (s.omega * s.dxdx + t[1] / s.omega);
+    bool accepted = false;
+    if (movement == 0.0) {
+      accepted = true;
+    } else {
+      const double lim = interaction > 0.0 ? movement / interaction : __builtin_inf();
+      const double tt = (double)(s.attempts + 1);
+      const double nxt =
+          fmax(fmin((1.0 - pow_call(tt, -0.3)) * lim, (1.0 + pow_call(tt, -0.6)) * s.eta), 1e-14);
+      if (s.eta <= lim) accepted = true;
+      else ++s.rejected;
+      s.eta = nxt;
+    }
+    if (accepted) {
+      s.next_mode = next_mode(s, s.k + 1);
+      return PH_B_ADAPT;
+    }
+    ++s.trial;
+    if (s.trial >= 60) {
+      s.status = ST_NUMERICAL;
+      return PH_FINAL;
+    }
+    ++s.attempts;
+    return PH_P_TRIAL;
   }
 
   // after a B phase: the iteration is complete (pdlp.py:401-403)
@@ -1192,33 +1226,7 @@ struct Transition {
         s.primal_ready = 0;
         return loop_top(s);
       case PH_P_TRIAL: s.dxdx = t[0]; return PH_A_TRIAL;
-      case PH_A_TRIAL: {
-        const double interaction = fabs(t[0]);
-        const double movement = 0.5 * (s.omega * s.dxdx + t[1] / s.omega);
-        bool accepted = false;
-        if (movement == 0.0) {
-          accepted = true;
-        } else {
-          const double lim = interaction > 0.0 ? movement / interaction : __builtin_inf();
-          const double tt = (double)(s.attempts + 1);
-          const double nxt =
-              fmax(fmin((1.0 - pow(tt, -0.3)) * lim, (1.0 + pow(tt, -0.6)) * s.eta), 1e-14);
-          if (s.eta <= lim) accepted = true;
-          else ++s.rejected;
-          s.eta = nxt;
-        }
-        if (accepted) {
-          s.next_mode = next_mode(s, s.k + 1);
-          return PH_B_ADAPT;
-        }
-        ++s.trial;
-        if (s.trial >= 60) {
-          s.status = ST_NUMERICAL;
-          return PH_FINAL;
-        }
-        ++s.attempts;
-        return PH_P_TRIAL;
-      }
+      case PH_A_TRIAL: return trial_step(s, t);
       case PH_B_ADAPT: return end_iteration(s, t[0]);
       case PH_A_FUSED: return PH_B_FUSED;
       case PH_B_FUSED: return end_iteration(s, t[0]);
