@@ -660,8 +660,39 @@ struct PhaseWork {
           next_primal(j, xj, aty, nm, tt, dxdx);
         };
         if (has_res) {
+          // the row sweep's operands are loaded before the column pass, so that
+          // their latency overlaps it (the sweep and the pass touch disjoint
+          // arrays: the results are the same in either order)
+          constexpr int kSw = 2;
+          T yn_[kSw], ax_[kSw], adx_[kSw], sy_[kSw];
+          const bool early = i1 - i0 <= (long long)kSw * BLOCK;
+          if (early) {
+#pragma unroll
+            for (int k = 0; k < kSw; ++k) {
+              const long long i = i0 + tid + (long long)k * BLOCK;
+              if (i < i1) {
+                yn_[k] = l.yn[i];
+                ax_[k] = l.ax[i];
+                adx_[k] = l.adx[i];
+                sy_[k] = l.sumy[i];
+              }
+            }
+          }
           tile_pass_res<BLOCK, T>(res.valT, res.idxT, res.ptrT, res.tilesT, res.dy, b0, b1,
                                   pre, epi);
+          if (early) {
+#pragma unroll
+            for (int k = 0; k < kSw; ++k) {
+              const long long i = i0 + tid + (long long)k * BLOCK;
+              if (i < i1) {
+                l.y[i] = yn_[k];
+                l.ax[i] = ax_[k] + adx_[k];
+                l.sumy[i] = sy_[k] + et * yn_[k];
+              }
+            }
+            part[0] += dxdx;
+            break;
+          }
         } else if (has_vec) {
           const LdsGather<T> ld{l.ti, l.stv, res.dy};
           const LdsBounds bd{res.rowsT, res.nzsT};
