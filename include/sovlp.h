@@ -408,9 +408,8 @@ int sovlp_batch_geometry(int precision, int block, int *tile_rows, int *tile_nnz
  * block's geometry; see sovlp_params.flags): *resident = 1 its scaled matrix
  * and step vectors, 2 only its step vectors dx and dy and its tile bounds (the
  * products then gather from LDS, the matrix streams from memory), 0 nothing.
- * No reference
- * counterpart (the reference streams the matrix from memory in every
- * product, backend.py:118-152). */
+ * No reference counterpart (the reference streams the matrix from memory in
+ * every product, backend.py:118-152). */
 int sovlp_lds_resident(int precision, int block, int m, int n, int64_t nnz, int tiles_a,
                        int tiles_t, int *resident);
 
