@@ -493,9 +493,24 @@ __device__ __forceinline__ T lds_row_sum(const SOV_LDS T* __restrict__ val,
     s = s + p2;
     s = s + p3;
   }
-  for (; q < b; q += step) {
-    const T p = val[q] * g[idx[q]];
-    s = s + p;
+  if (q < b) {
+    // the last 1-3 products: their reads issued together from in-range
+    // slots (clamped to q), each product added only if it is the row's
+    int jr[3];
+    T vr[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int qr = q + r * step < b ? q + r * step : q;
+      jr[r] = idx[qr];
+      vr[r] = val[qr];
+    }
+    T pr[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) pr[r] = vr[r] * g[jr[r]];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      if (q + r * step < b) s = s + pr[r];
+    }
   }
   return s;
 }
